@@ -1,0 +1,167 @@
+/*
+ * scs.h - C ABI of the MI355X-native size-bounded community search engine.
+ *
+ * The reference (wajidmanzoor/SCS) has no plugin or FFI layer: its host
+ * server calls its CUDA kernels directly from main.cu.  This header is the
+ * seam that replaces that host->device boundary:
+ *
+ *   reference                                  this ABI
+ *   ---------------------------------------    --------------------------
+ *   load_graph + core_decomposition_linear_    scs_graph_load /
+ *     list (src/Graph.h:5-109),                scs_graph_create
+ *     memoryAllocationGenGraph (main.cu:535)
+ *   memeoryAllocationGraph/Task/Buffer         scs_engine_create
+ *     (main.cu:536-563), CLI args 2-12
+ *     (main.cu:481-502)
+ *   processMessages admission: CSSC_heu,       scs_submit
+ *     cal_query_dist, initialReductionRules,
+ *     CompressTask, NeighborUpdate
+ *     (main.cu:84-240)
+ *   one level of ProcessTask/FindDoms/Expand/  scs_step
+ *     RemoveCompletedTasks (main.cu:261-442)
+ *   per-slot completion test and results       scs_poll
+ *     line (main.cu:317-360, 444-463)
+ *   freeGenGraph/freeGraph/... (main.cu:588)   scs_engine_destroy,
+ *                                              scs_graph_destroy
+ *
+ * All functions return 0 on success or a negative SCS_E* code; none exits
+ * the process.  Pointers are plain host pointers; no torch types appear.
+ * One host thread drives an engine; scs_submit may be called from any
+ * thread (it only appends to a locked host queue).
+ */
+#ifndef SCS_H_
+#define SCS_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCS_OK 0
+#define SCS_EINVAL (-1)   /* bad argument (malformed query, h == 0, q >= n) */
+#define SCS_ENOMEM (-2)   /* device or host allocation failed */
+#define SCS_EAGAIN (-3)   /* admission refused for now: slots or memory full */
+#define SCS_EHIP (-4)     /* a HIP runtime call failed */
+#define SCS_EIO (-5)      /* graph file missing or unreadable */
+#define SCS_EBIG (-6)     /* query needs more than the engine's per-query limits */
+
+#define SCS_MAX_H 1000    /* largest upper size bound h accepted */
+#define SCS_MAX_DOMS 32   /* largest limit_dominating_set accepted */
+
+/* Result status (reference "Overtime"/"Heu" columns, main.cu:123-128,328-333,
+ * 351, 454, extended with explicit per-query outcomes). */
+enum scs_status {
+  SCS_ST_OK = 0,         /* search complete: kl is the optimum */
+  SCS_ST_HEU_OPT = 1,    /* heuristic reached ku = min(core[q], h-1): optimal */
+  SCS_ST_CAPPED = 2,     /* node budget exhausted: kl is a lower bound */
+  SCS_ST_INFEASIBLE = 3, /* no connected H with q and l <= |H| <= h exists */
+  SCS_ST_REJECTED = 4    /* kept set larger than the engine's per-query limit */
+};
+
+typedef struct scs_graph scs_graph;
+typedef struct scs_engine scs_engine;
+
+/* Engine parameters.  The first twelve mirror the reference's positional
+ * command line (main.cu:481-502) with the same meaning where the design
+ * keeps the concept; see each comment. */
+typedef struct scs_params {
+  uint32_t partition_size; /* argv[2]: words of depth-first state per wave
+                              (0 = sized from the largest kept set) */
+  uint32_t buffer_size;    /* argv[3]: words of the shared task pool
+                              (0 = derived from limit_tasks) */
+  double copy_limit;       /* argv[4]: a wave donates work while fewer than
+                              copy_limit * waves tasks are pooled */
+  uint32_t read_limit;     /* argv[5]: kept for the command line; one task is
+                              taken per visit to the pool */
+  uint32_t limit_queries;  /* argv[6]: resident query slots */
+  uint32_t factor;         /* argv[7]: accepted for the command line; task
+                              offsets are not compressed here */
+  uint32_t red1, red2, red3;  /* argv[8..10]: reduction rules 1-3 */
+  uint32_t prun1, prun2;      /* argv[11..12]: bounds U_d and U_nr */
+  /* named extensions */
+  uint32_t waves_per_cu;   /* resident search waves per CU (0 = 4) */
+  uint32_t limit_tasks;    /* task pool slots (0 = 4 * waves) */
+  uint64_t node_budget;    /* per-query node cap before CAPPED (0 = none) */
+  uint64_t mem_budget;     /* bytes of device memory for per-query graphs */
+  uint32_t max_kept;       /* per-query kept-set cap (0 = graph size) */
+  uint32_t time_slice_us;  /* length of one scs_step kernel epoch */
+  uint32_t deterministic;  /* 1 = a single wave and no work sharing: node
+                              counts equal the CPU oracle's */
+  uint32_t learned;        /* 1 = MLP scores the branching vertex */
+  const void* model;       /* MLP weights (scs_model layout) when learned */
+  uint32_t heuristic_threads; /* host threads for CSSC_heu (0 = 4) */
+  uint32_t reserved[7];
+} scs_params;
+
+typedef struct scs_result {
+  uint64_t tag;          /* caller's tag from scs_submit */
+  uint32_t l, h, q;      /* the query */
+  uint32_t kl;           /* maximum minimum degree found */
+  uint32_t heu_kl;       /* initial lower bound from the heuristic */
+  uint32_t ku;           /* min(core[q], h-1) */
+  int32_t status;        /* enum scs_status */
+  uint32_t kept;         /* |K| after per-query preprocessing */
+  uint64_t nodes;        /* branch-and-bound nodes expanded */
+  double t_submit_us;    /* host clock at scs_submit */
+  double t_done_us;      /* host clock when the result was collected */
+  uint32_t wsize;        /* |H|, 0 when no witness */
+  uint32_t witness[SCS_MAX_H]; /* H (original vertex ids) */
+} scs_result;
+
+/* Graph: builds the CSR on the device and computes core numbers there.
+ * off: n+1 offsets (int64), adj: off[n] neighbour ids (int32), symmetric,
+ * no self-loops or duplicates.  flags: reserved, pass 0. */
+int scs_graph_create(const int64_t* off, const int32_t* adj, uint32_t n, uint64_t nnz,
+                     uint32_t flags, scs_graph** out);
+/* Reference edge-list loader semantics (src/Graph.h:5-67): header "n m",
+ * whitespace-separated pairs, self-loops dropped, duplicates merged,
+ * adjacency ascending; ids >= n are SCS_EINVAL. */
+int scs_graph_load(const char* path, scs_graph** out);
+int scs_graph_info(const scs_graph* g, uint32_t* n, uint64_t* nnz, uint32_t* dmax);
+/* Copies core numbers (n values) and the CSR (off n+1, adj nnz) to the host. */
+int scs_graph_core(const scs_graph* g, uint32_t* core);
+int scs_graph_csr(const scs_graph* g, int64_t* off, int32_t* adj);
+/* Device time of the core decomposition in microseconds. */
+double scs_graph_core_time_us(const scs_graph* g);
+void scs_graph_destroy(scs_graph* g);
+
+void scs_params_default(scs_params* p);
+int scs_engine_create(scs_graph* g, const scs_params* p, scs_engine** out);
+/* Queues a query "<l> <h> <q> <heuristic_flag> <limit_dominating_set>".
+ * Returns SCS_OK (queued), SCS_EINVAL. */
+int scs_submit(scs_engine* e, uint32_t l, uint32_t h, uint32_t q, uint32_t heu,
+               uint32_t limit_doms, uint64_t tag);
+/* Parses the reference's query text (main.cu:92-109) and submits it. */
+int scs_submit_text(scs_engine* e, const char* text, uint64_t tag);
+/* Admits queued queries while slots and memory allow, runs one kernel
+ * epoch and collects finished queries.  Returns the number of queries
+ * still queued or in flight (>= 0) or a negative error. */
+int scs_step(scs_engine* e);
+/* Copies up to max finished results; returns how many were copied. */
+int scs_poll(scs_engine* e, scs_result* out, int max);
+/* Runs scs_step until nothing is queued or in flight. */
+int scs_drain(scs_engine* e);
+void scs_engine_destroy(scs_engine* e);
+
+/* Engine counters. */
+typedef struct scs_stats {
+  uint64_t nodes;          /* total nodes expanded */
+  uint64_t donations;      /* tasks handed to the shared pool */
+  uint64_t rebuilds;       /* wave state rebuilds from pooled tasks */
+  double engine_us;        /* device time inside search epochs */
+  double prep_us;          /* device time inside per-query preprocessing */
+  double heu_us;           /* host time inside CSSC_heu */
+  uint64_t prep_bytes;     /* algorithmic bytes of preprocessing */
+  uint64_t epochs;
+  uint32_t waves;
+  uint32_t kcap;
+} scs_stats;
+int scs_engine_stats(const scs_engine* e, scs_stats* s);
+
+const char* scs_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCS_H_ */

@@ -1,0 +1,863 @@
+// C ABI of the SCS engine: graph upload and core decomposition, per-query
+// admission (heuristic, GPU preprocessing), search epochs and results.
+// Mirrors the reference host server's responsibilities (main.cu:80-597)
+// without its global state, busy loops or process exits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "scs_host.h"
+#include "scs_internal.h"
+
+extern "C" {
+hipError_t scs_launch_core_init(const i64* off, u32 n, u32* deg, u32* core, hipStream_t s);
+hipError_t scs_launch_core_scan(u32 n, u32 k, const u32* deg, u32* core, u32* F, u32* fcount,
+                                u32* minrest, hipStream_t s);
+hipError_t scs_launch_core_peel(const i64* off, const int32_t* adj, u32 k, const u32* F, u32 fn,
+                                u32* deg, u32* core, u32* Fn, u32* fncount, hipStream_t s);
+hipError_t scs_launch_bfs_seed(u32 q, u32 id, u32* stamp, u32* dist, u32* F, u32* fcount,
+                               hipStream_t s);
+hipError_t scs_launch_bfs_level(const i64* off, const int32_t* adj, const u32* core, u32 kl0,
+                                u32 id, u32 level, const u32* F, u32 fn, u32* stamp, u32* dist,
+                                u32* Fn, u32* fncount, hipStream_t s);
+hipError_t scs_launch_kept_scan(u32 n, const u32* stamp, u32 id, u32* bsum, u32* locid,
+                                u32* total, hipStream_t s);
+hipError_t scs_launch_scan_u32(const u32* in, u32 n, u32* bsum, u32* out, u32* total,
+                               hipStream_t s);
+hipError_t scs_launch_kept_order(u32 n, const u32* stamp, u32 id, const u32* locid, u32* korder,
+                                 hipStream_t s);
+hipError_t scs_launch_local_degree(const i64* off, const int32_t* adj, const u32* stamp, u32 id,
+                                   const u32* korder, u32 K, u32* ldeg, hipStream_t s);
+hipError_t scs_launch_local_fill(const i64* off, const int32_t* adj, const u32* stamp, u32 id,
+                                 const u32* locid, const u32* korder, u32 K, const u32* loff,
+                                 const u32* dist, const u32* core, u32* ladj, u32* ldist,
+                                 u32* lcore, u32* lorig, hipStream_t s);
+hipError_t scs_launch_prep_bytes(const i64* off, const u32* korder, u32 K, unsigned long long* out,
+                                 hipStream_t s);
+hipError_t scs_launch_epoch(const EngineArgs* A, hipStream_t s);
+hipError_t scs_launch_admit(const EngineArgs* A, const u32* slots, u32 n, u32* failed,
+                            hipStream_t s);
+hipError_t scs_launch_pool_init(u64* seq, u32 n, hipStream_t s);
+}
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      fprintf(stderr, "scs: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+              __LINE__);                                                            \
+      return SCS_EHIP;                                                              \
+    }                                                                               \
+  } while (0)
+
+static double now_us() {
+  using namespace std::chrono;
+  return duration<double, std::micro>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+// Graph
+// ---------------------------------------------------------------------------
+struct scs_graph {
+  scs::HostGraph hg;
+  i64* d_off = nullptr;
+  int32_t* d_adj = nullptr;
+  u32* d_core = nullptr;
+  double core_us = 0;
+};
+
+static int core_decompose(scs_graph* G) {
+  const u32 n = G->hg.n;
+  hipStream_t s;
+  HIPCHK(hipStreamCreate(&s));
+  u32 *deg, *F, *Fn, *cnt;
+  HIPCHK(hipMalloc(&deg, sizeof(u32) * std::max<u32>(n, 1)));
+  HIPCHK(hipMalloc(&F, sizeof(u32) * std::max<u32>(n, 1)));
+  HIPCHK(hipMalloc(&Fn, sizeof(u32) * std::max<u32>(n, 1)));
+  HIPCHK(hipMalloc(&cnt, sizeof(u32) * 4));
+  u32* h_cnt;
+  HIPCHK(hipHostMalloc(&h_cnt, sizeof(u32) * 4));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  HIPCHK(scs_launch_core_init(G->d_off, n, deg, G->d_core, s));
+  u32 peeled = 0, k = 0;
+  while (peeled < n) {
+    // cnt[0] = frontier size, cnt[1] = min remaining degree
+    u32 init[2] = {0, 0xffffffffu};
+    HIPCHK(hipMemcpyAsync(cnt, init, sizeof(init), hipMemcpyHostToDevice, s));
+    HIPCHK(scs_launch_core_scan(n, k, deg, G->d_core, F, cnt, cnt + 1, s));
+    HIPCHK(hipMemcpyAsync(h_cnt, cnt, 2 * sizeof(u32), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    u32 fn = h_cnt[0], minrest = h_cnt[1];
+    if (fn == 0) {
+      if (minrest == 0xffffffffu) break;
+      k = std::max(k + 1, minrest);
+      continue;
+    }
+    peeled += fn;
+    u32 *cur = F, *nxt = Fn;
+    while (fn) {
+      HIPCHK(hipMemsetAsync(cnt + 2, 0, sizeof(u32), s));
+      HIPCHK(scs_launch_core_peel(G->d_off, G->d_adj, k, cur, fn, deg, G->d_core, nxt, cnt + 2, s));
+      HIPCHK(hipMemcpyAsync(h_cnt + 2, cnt + 2, sizeof(u32), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      fn = h_cnt[2];
+      peeled += fn;
+      std::swap(cur, nxt);
+    }
+    k++;
+  }
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  G->core_us = ms * 1000.0;
+  G->hg.core.resize(n);
+  HIPCHK(hipMemcpy(G->hg.core.data(), G->d_core, sizeof(u32) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipFree(deg));
+  HIPCHK(hipFree(F));
+  HIPCHK(hipFree(Fn));
+  HIPCHK(hipFree(cnt));
+  HIPCHK(hipHostFree(h_cnt));
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  HIPCHK(hipStreamDestroy(s));
+  return SCS_OK;
+}
+
+static int graph_upload(scs_graph* G) {
+  const u32 n = G->hg.n;
+  const u64 nnz = (u64)G->hg.off[n];
+  HIPCHK(hipMalloc(&G->d_off, sizeof(i64) * (n + 1)));
+  HIPCHK(hipMalloc(&G->d_adj, sizeof(int32_t) * std::max<u64>(nnz, 1)));
+  HIPCHK(hipMalloc(&G->d_core, sizeof(u32) * std::max<u32>(n, 1)));
+  HIPCHK(hipMemcpy(G->d_off, G->hg.off.data(), sizeof(i64) * (n + 1), hipMemcpyHostToDevice));
+  if (nnz)
+    HIPCHK(hipMemcpy(G->d_adj, G->hg.adj.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+  G->hg.dmax = 0;
+  for (u32 v = 0; v < n; v++) G->hg.dmax = std::max<u32>(G->hg.dmax, (u32)(G->hg.off[v + 1] - G->hg.off[v]));
+  return core_decompose(G);
+}
+
+extern "C" int scs_graph_create(const int64_t* off, const int32_t* adj, uint32_t n, uint64_t nnz,
+                                uint32_t flags, scs_graph** out) {
+  (void)flags;
+  if (!off || !out || (nnz && !adj)) return SCS_EINVAL;
+  if (off[0] != 0 || (u64)off[n] != nnz) return SCS_EINVAL;
+  for (u32 v = 0; v < n; v++)
+    if (off[v + 1] < off[v]) return SCS_EINVAL;
+  for (u64 j = 0; j < nnz; j++)
+    if (adj[j] < 0 || (u32)adj[j] >= n) return SCS_EINVAL;
+  scs_graph* G = new scs_graph;
+  G->hg.n = n;
+  G->hg.off.assign(off, off + n + 1);
+  G->hg.adj.assign(adj, adj + nnz);
+  int rc = graph_upload(G);
+  if (rc) {
+    scs_graph_destroy(G);
+    return rc;
+  }
+  *out = G;
+  return SCS_OK;
+}
+
+// Reference loader semantics (src/Graph.h:5-67).
+extern "C" int scs_graph_load(const char* path, scs_graph** out) {
+  if (!path || !out) return SCS_EINVAL;
+  FILE* f = fopen(path, "rb");
+  if (!f) return SCS_EIO;
+  std::string buf;
+  {
+    char tmp[1 << 16];
+    size_t r;
+    while ((r = fread(tmp, 1, sizeof(tmp), f)) > 0) buf.append(tmp, r);
+  }
+  fclose(f);
+  const char* p = buf.c_str();
+  const char* end = p + buf.size();
+  auto next = [&](u64& x) -> bool {
+    while (p < end && (*p < '0' || *p > '9')) {
+      if (*p == '-') return false;
+      p++;
+    }
+    if (p >= end) return false;
+    x = 0;
+    while (p < end && *p >= '0' && *p <= '9') x = x * 10 + (u64)(*p++ - '0');
+    return true;
+  };
+  u64 n, m;
+  if (!next(n) || !next(m) || n > 0xfffffffeull) return SCS_EIO;
+  std::vector<u64> keys;
+  keys.reserve(2 * m + 16);
+  u64 a, b;
+  while (next(a)) {
+    if (!next(b)) break;
+    if (a >= n || b >= n) return SCS_EINVAL;
+    if (a == b) continue;
+    keys.push_back((a << 32) | b);
+    keys.push_back((b << 32) | a);
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  scs_graph* G = new scs_graph;
+  G->hg.n = (u32)n;
+  G->hg.off.assign(n + 1, 0);
+  G->hg.adj.resize(keys.size());
+  for (size_t i = 0; i < keys.size(); i++) {
+    G->hg.off[(keys[i] >> 32) + 1]++;
+    G->hg.adj[i] = (int32_t)(keys[i] & 0xffffffffu);
+  }
+  for (u64 i = 0; i < n; i++) G->hg.off[i + 1] += G->hg.off[i];
+  int rc = graph_upload(G);
+  if (rc) {
+    scs_graph_destroy(G);
+    return rc;
+  }
+  *out = G;
+  return SCS_OK;
+}
+
+extern "C" int scs_graph_info(const scs_graph* g, uint32_t* n, uint64_t* nnz, uint32_t* dmax) {
+  if (!g) return SCS_EINVAL;
+  if (n) *n = g->hg.n;
+  if (nnz) *nnz = (u64)g->hg.off[g->hg.n];
+  if (dmax) *dmax = g->hg.dmax;
+  return SCS_OK;
+}
+extern "C" int scs_graph_core(const scs_graph* g, uint32_t* core) {
+  if (!g || !core) return SCS_EINVAL;
+  std::copy(g->hg.core.begin(), g->hg.core.end(), core);
+  return SCS_OK;
+}
+extern "C" int scs_graph_csr(const scs_graph* g, int64_t* off, int32_t* adj) {
+  if (!g) return SCS_EINVAL;
+  if (off) std::copy(g->hg.off.begin(), g->hg.off.end(), off);
+  if (adj) std::copy(g->hg.adj.begin(), g->hg.adj.end(), adj);
+  return SCS_OK;
+}
+extern "C" double scs_graph_core_time_us(const scs_graph* g) { return g ? g->core_us : 0.0; }
+extern "C" void scs_graph_destroy(scs_graph* g) {
+  if (!g) return;
+  if (g->d_off) (void)hipFree(g->d_off);
+  if (g->d_adj) (void)hipFree(g->d_adj);
+  if (g->d_core) (void)hipFree(g->d_core);
+  delete g;
+}
+
+// ---------------------------------------------------------------------------
+// Device arena for per-query local graphs (first fit, 256-byte granules).
+// ---------------------------------------------------------------------------
+struct Arena {
+  u8* base = nullptr;
+  u64 size = 0;
+  std::map<u64, u64> free_;  // offset -> length
+  void init(u8* b, u64 s) {
+    base = b;
+    size = s;
+    free_.clear();
+    free_[0] = s;
+  }
+  bool alloc(u64 bytes, u64* off) {
+    bytes = (bytes + 255) & ~255ull;
+    if (bytes == 0) bytes = 256;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        u64 rest = it->second - bytes;
+        u64 noff = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[noff] = rest;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(u64 off, u64 bytes) {
+    bytes = (bytes + 255) & ~255ull;
+    if (bytes == 0) bytes = 256;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+struct PendingQuery {
+  uint64_t tag;
+  u32 l, h, q, heu, ld;
+  double t_submit;
+};
+
+struct HostSlot {
+  bool used = false;
+  PendingQuery pq;
+  u32 kl0 = 0, ku = 0, K = 0;
+  u64 a_off = 0, a_len = 0, b_off = 0, b_len = 0;
+};
+
+struct scs_engine {
+  scs_graph* g = nullptr;
+  scs_params p{};
+  hipStream_t s = nullptr;
+  u32 nwaves = 0, kcap = 0;
+  // device
+  QuerySlot* d_slots = nullptr;
+  u32* d_witness = nullptr;
+  u8* d_wave = nullptr;
+  u64* d_pool_seq = nullptr;
+  u32* d_pool_data = nullptr;
+  u64* d_pool_pos = nullptr;
+  EngineGlobals* d_glob = nullptr;
+  EngineArgs args{};
+  u32 *d_stamp = nullptr, *d_dist = nullptr, *d_F = nullptr, *d_Fn = nullptr;
+  u32 *d_locid = nullptr, *d_korder = nullptr, *d_ldeg = nullptr, *d_bsum = nullptr;
+  u32* d_cnt = nullptr;
+  u32* d_admit = nullptr;
+  unsigned long long* d_prepbytes = nullptr;
+  u8* d_arena = nullptr;
+  Arena arena;
+  u32 stamp_id = 0;
+  u32* h_cnt = nullptr;  // pinned
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, pe0 = nullptr, pe1 = nullptr;
+  // host
+  std::mutex qmu;
+  std::deque<PendingQuery> queue;
+  std::vector<HostSlot> slots;
+  std::vector<QuerySlot> h_slots;
+  std::vector<scs_result> results;
+  std::vector<scs::HeuWork> heu_work;
+  u32 active = 0;
+  scs_stats st{};
+};
+
+extern "C" void scs_params_default(scs_params* p) {
+  memset(p, 0, sizeof(*p));
+  p->partition_size = 0;
+  p->buffer_size = 0;
+  p->copy_limit = 0.5;
+  p->read_limit = 1;
+  p->limit_queries = 64;
+  p->factor = 1;
+  p->red1 = p->red2 = p->red3 = 1;
+  p->prun1 = p->prun2 = 1;
+  p->waves_per_cu = 4;
+  p->limit_tasks = 0;
+  p->node_budget = 0;
+  p->mem_budget = 1ull << 30;
+  p->max_kept = 0;
+  p->time_slice_us = 2000;
+  p->deterministic = 0;
+  p->learned = 0;
+  p->model = nullptr;
+  p->heuristic_threads = 4;
+}
+
+static u64 align256(u64 x) { return (x + 255) & ~255ull; }
+
+static WaveLayout make_layout(u32 kcap) {
+  WaveLayout L{};
+  u64 o = 0;
+  const u64 k4 = (u64)((kcap + 3) & ~3u);
+  L.st = o; o = align256(o + k4);
+  L.nmark = o; o = align256(o + k4);
+  L.queued = o; o = align256(o + k4);
+  L.dC = o; o = align256(o + 2 * k4);
+  L.dR = o; o = align256(o + 4 * k4);
+  L.pos = o; o = align256(o + 4 * k4);
+  L.rlist = o; o = align256(o + 4 * k4);
+  L.acc = o; o = align256(o + 8 * k4);
+  L.work = o; o = align256(o + 4 * (k4 + 64));
+  L.front = o; o = align256(o + 4 * (k4 + 64));
+  L.trail = o; o = align256(o + 4 * (k4 + 64));
+  L.frames = o; o = align256(o + sizeof(Frame) * (SCS_HCAP + 1));
+  L.clist = o; o = align256(o + 4 * (SCS_HCAP + 64));
+  L.hist = o; o = align256(o + 4 * (SCS_HCAP + 4));
+  L.hdr = o; o = align256(o + sizeof(WaveHdr));
+  L.stride = o;
+  return L;
+}
+
+extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine** out) {
+  if (!g || !out) return SCS_EINVAL;
+  scs_params p;
+  if (pin)
+    p = *pin;
+  else
+    scs_params_default(&p);
+  if (p.limit_queries == 0) p.limit_queries = 1;
+  if (p.waves_per_cu == 0) p.waves_per_cu = 4;
+  if (p.time_slice_us == 0) p.time_slice_us = 2000;
+  if (p.heuristic_threads == 0) p.heuristic_threads = 4;
+  scs_engine* E = new scs_engine;
+  E->g = g;
+  E->p = p;
+  auto fail = [&](int rc) {
+    scs_engine_destroy(E);
+    return rc;
+  };
+  if (hipStreamCreate(&E->s) != hipSuccess) return fail(SCS_EHIP);
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return fail(SCS_EHIP);
+  const u32 n = g->hg.n;
+  E->nwaves = p.deterministic ? 1u : (u32)prop.multiProcessorCount * p.waves_per_cu;
+  E->kcap = p.max_kept ? std::min(p.max_kept, n) : n;
+  if (E->kcap == 0) E->kcap = 1;
+  if (p.partition_size && p.partition_size < E->kcap) E->kcap = std::max<u32>(p.partition_size, 1);
+  const WaveLayout lay = make_layout(E->kcap);
+  // pool
+  u32 donate_below = p.deterministic ? 0u : (u32)std::max(1.0, p.copy_limit * E->nwaves);
+  u32 nslots = p.limit_tasks ? p.limit_tasks : 2 * E->nwaves;
+  nslots = std::max<u32>(nslots, donate_below + p.limit_queries + 64);
+  u32 slot_words = 4 + SCS_HCAP + (E->kcap + 1) / 2 + 64;
+  if (p.buffer_size) nslots = std::max<u32>(p.limit_queries + 64, p.buffer_size / slot_words);
+  const u32 nq = p.limit_queries;
+  // allocations
+  if (hipMalloc(&E->d_slots, sizeof(QuerySlot) * nq) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_witness, sizeof(u32) * SCS_MAX_H * nq) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_wave, lay.stride * E->nwaves) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_pool_seq, sizeof(u64) * nslots) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_pool_data, sizeof(u32) * (u64)slot_words * nslots) != hipSuccess)
+    return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_pool_pos, sizeof(u64) * 2) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_glob, sizeof(EngineGlobals)) != hipSuccess) return fail(SCS_ENOMEM);
+  const u32 n1 = std::max<u32>(n, 1);
+  if (hipMalloc(&E->d_stamp, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_dist, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_F, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_Fn, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_locid, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_korder, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_ldeg, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_bsum, sizeof(u32) * (n1 / 2048 + 64)) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_cnt, sizeof(u32) * 16) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_admit, sizeof(u32) * 2 * nq) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_prepbytes, sizeof(unsigned long long)) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_arena, p.mem_budget) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipHostMalloc(&E->h_cnt, sizeof(u32) * 16) != hipSuccess) return fail(SCS_ENOMEM);
+  E->arena.init(E->d_arena, p.mem_budget);
+  if (hipMemset(E->d_slots, 0, sizeof(QuerySlot) * nq) != hipSuccess) return fail(SCS_EHIP);
+  if (hipMemset(E->d_wave, 0, lay.stride * E->nwaves) != hipSuccess) return fail(SCS_EHIP);
+  if (hipMemset(E->d_pool_pos, 0, sizeof(u64) * 2) != hipSuccess) return fail(SCS_EHIP);
+  if (hipMemset(E->d_glob, 0, sizeof(EngineGlobals)) != hipSuccess) return fail(SCS_EHIP);
+  if (hipMemset(E->d_stamp, 0, sizeof(u32) * n1) != hipSuccess) return fail(SCS_EHIP);
+  if (hipMemset(E->d_prepbytes, 0, sizeof(unsigned long long)) != hipSuccess) return fail(SCS_EHIP);
+  if (scs_launch_pool_init(E->d_pool_seq, nslots, E->s) != hipSuccess) return fail(SCS_EHIP);
+  if (hipStreamSynchronize(E->s) != hipSuccess) return fail(SCS_EHIP);
+  for (hipEvent_t* ev : {&E->ev0, &E->ev1, &E->pe0, &E->pe1})
+    if (hipEventCreate(ev) != hipSuccess) return fail(SCS_EHIP);
+  EngineArgs& A = E->args;
+  A.slots = E->d_slots;
+  A.nslots = nq;
+  A.wave_mem = E->d_wave;
+  A.lay = lay;
+  A.kcap = E->kcap;
+  A.nwaves = E->nwaves;
+  A.pool.seq = E->d_pool_seq;
+  A.pool.data = E->d_pool_data;
+  A.pool.enq = E->d_pool_pos;
+  A.pool.deq = E->d_pool_pos + 1;
+  A.pool.nslots = nslots;
+  A.pool.slot_words = slot_words;
+  A.glob = E->d_glob;
+  A.slice_ticks = (u64)p.time_slice_us * 100ull;  // wall_clock64 runs at 100 MHz
+  A.donate_below = donate_below;
+  A.deterministic = p.deterministic;
+  A.node_flush = p.deterministic ? 1 : 16;
+  E->slots.resize(nq);
+  E->h_slots.resize(nq);
+  E->heu_work.resize(p.heuristic_threads);
+  E->st.waves = E->nwaves;
+  E->st.kcap = E->kcap;
+  *out = E;
+  return SCS_OK;
+}
+
+extern "C" void scs_engine_destroy(scs_engine* E) {
+  if (!E) return;
+  if (E->s) (void)hipStreamSynchronize(E->s);
+  void* ptrs[] = {E->d_slots, E->d_witness, E->d_wave, E->d_pool_seq, E->d_pool_data,
+                  E->d_pool_pos, E->d_glob, E->d_stamp, E->d_dist, E->d_F, E->d_Fn, E->d_locid,
+                  E->d_korder, E->d_ldeg, E->d_bsum, E->d_cnt, E->d_admit, E->d_prepbytes,
+                  E->d_arena};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (E->h_cnt) (void)hipHostFree(E->h_cnt);
+  for (hipEvent_t ev : {E->ev0, E->ev1, E->pe0, E->pe1})
+    if (ev) (void)hipEventDestroy(ev);
+  if (E->s) (void)hipStreamDestroy(E->s);
+  delete E;
+}
+
+extern "C" int scs_submit(scs_engine* E, uint32_t l, uint32_t h, uint32_t q, uint32_t heu,
+                          uint32_t limit_doms, uint64_t tag) {
+  if (!E) return SCS_EINVAL;
+  if (h == 0 || h > SCS_MAX_H || q >= E->g->hg.n || limit_doms > SCS_MAX_DOMS) return SCS_EINVAL;
+  std::lock_guard<std::mutex> lk(E->qmu);
+  E->queue.push_back(PendingQuery{tag, l, h, q, heu, limit_doms, now_us()});
+  return SCS_OK;
+}
+
+// Query text "<l> <h> <q> <heu> <limitDoms>" (main.cu:92-109).
+extern "C" int scs_submit_text(scs_engine* E, const char* text, uint64_t tag) {
+  if (!E || !text) return SCS_EINVAL;
+  u64 v[6];
+  int c = 0;
+  const char* p = text;
+  while (*p && c < 6) {
+    while (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r') p++;
+    if (!*p) break;
+    if (*p < '0' || *p > '9') return SCS_EINVAL;
+    u64 x = 0;
+    while (*p >= '0' && *p <= '9') {
+      x = x * 10 + (u64)(*p - '0');
+      if (x > 0xffffffffull) return SCS_EINVAL;
+      p++;
+    }
+    if (*p && *p != ' ' && *p != '\t' && *p != '\n' && *p != '\r') return SCS_EINVAL;
+    v[c++] = x;
+  }
+  while (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r') p++;
+  if (c != 5 || *p) return SCS_EINVAL;
+  return scs_submit(E, (u32)v[0], (u32)v[1], (u32)v[2], (u32)v[3], (u32)v[4], tag);
+}
+
+static void push_result(scs_engine* E, const PendingQuery& pq, u32 kl, u32 heu_kl, u32 ku,
+                        int status, u32 K, u64 nodes, const u32* wit, u32 wsize) {
+  scs_result r;
+  memset(&r, 0, sizeof(r) - sizeof(r.witness));
+  r.tag = pq.tag;
+  r.l = pq.l;
+  r.h = pq.h;
+  r.q = pq.q;
+  r.kl = kl;
+  r.heu_kl = heu_kl;
+  r.ku = ku;
+  r.status = status;
+  r.kept = K;
+  r.nodes = nodes;
+  r.t_submit_us = pq.t_submit;
+  r.t_done_us = now_us();
+  r.wsize = std::min<u32>(wsize, SCS_MAX_H);
+  if (wit && r.wsize) memcpy(r.witness, wit, sizeof(u32) * r.wsize);
+  E->results.push_back(r);
+}
+
+// GPU preprocessing of one query into slot si.  Returns SCS_OK, SCS_EBIG
+// (kept set above kcap) or SCS_EAGAIN (arena full).
+static int prep_query(scs_engine* E, u32 si, u32 kl0, u32 ubD) {
+  HostSlot& hs = E->slots[si];
+  const scs::HostGraph& hg = E->g->hg;
+  const u32 n = hg.n;
+  hipStream_t s = E->s;
+  if (++E->stamp_id == 0) {
+    HIPCHK(hipMemsetAsync(E->d_stamp, 0, sizeof(u32) * n, s));
+    E->stamp_id = 1;
+  }
+  const u32 id = E->stamp_id;
+  HIPCHK(hipEventRecord(E->pe0, s));
+  HIPCHK(scs_launch_bfs_seed(hs.pq.q, id, E->d_stamp, E->d_dist, E->d_F, E->d_cnt, s));
+  u32 *F = E->d_F, *Fn = E->d_Fn;
+  u32 *fc = E->d_cnt, *fnc = E->d_cnt + 1;
+  for (u32 level = 0; level < ubD; level++) {
+    HIPCHK(hipMemcpyAsync(E->h_cnt, fc, sizeof(u32), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const u32 fn = E->h_cnt[0];
+    if (fn == 0) break;
+    HIPCHK(hipMemsetAsync(fnc, 0, sizeof(u32), s));
+    HIPCHK(scs_launch_bfs_level(E->g->d_off, E->g->d_adj, E->g->d_core, kl0, id, level, F, fn,
+                                E->d_stamp, E->d_dist, Fn, fnc, s));
+    std::swap(F, Fn);
+    std::swap(fc, fnc);
+  }
+  HIPCHK(scs_launch_kept_scan(n, E->d_stamp, id, E->d_bsum, E->d_locid, E->d_cnt + 2, s));
+  HIPCHK(hipMemcpyAsync(E->h_cnt + 2, E->d_cnt + 2, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const u32 K = E->h_cnt[2];
+  hs.K = K;
+  if (K > E->kcap) return SCS_EBIG;
+  HIPCHK(scs_launch_kept_order(n, E->d_stamp, id, E->d_locid, E->d_korder, s));
+  HIPCHK(scs_launch_local_degree(E->g->d_off, E->g->d_adj, E->d_stamp, id, E->d_korder, K,
+                                 E->d_ldeg, s));
+  // block A: loff (K+1) | ldist (K) | lcore (K) | lorig (K)
+  hs.a_len = sizeof(u32) * (4ull * K + 1);
+  if (!E->arena.alloc(hs.a_len, &hs.a_off)) return SCS_EAGAIN;
+  u32* A = (u32*)(E->d_arena + hs.a_off);
+  u32 *loff = A, *ldist = A + K + 1, *lcore = ldist + K, *lorig = lcore + K;
+  HIPCHK(scs_launch_scan_u32(E->d_ldeg, K, E->d_bsum, loff, E->d_cnt + 3, s));
+  HIPCHK(hipMemcpyAsync(loff + K, E->d_cnt + 3, sizeof(u32), hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(E->h_cnt + 3, E->d_cnt + 3, sizeof(u32), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const u32 nnzK = E->h_cnt[3];
+  hs.b_len = sizeof(u32) * (u64)std::max<u32>(nnzK, 1);
+  if (!E->arena.alloc(hs.b_len, &hs.b_off)) {
+    E->arena.release(hs.a_off, hs.a_len);
+    hs.a_len = 0;
+    return SCS_EAGAIN;
+  }
+  u32* ladj = (u32*)(E->d_arena + hs.b_off);
+  HIPCHK(scs_launch_local_fill(E->g->d_off, E->g->d_adj, E->d_stamp, id, E->d_locid, E->d_korder,
+                               K, loff, E->d_dist, E->g->d_core, ladj, ldist, lcore, lorig, s));
+  HIPCHK(scs_launch_prep_bytes(E->g->d_off, E->d_korder, K, E->d_prepbytes, s));
+  HIPCHK(hipEventRecord(E->pe1, s));
+  HIPCHK(hipEventSynchronize(E->pe1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, E->pe0, E->pe1));
+  E->st.prep_us += ms * 1000.0;
+  // slot record
+  u32 qloc = 0;
+  HIPCHK(hipMemcpy(&qloc, E->d_locid + hs.pq.q, sizeof(u32), hipMemcpyDeviceToHost));
+  QuerySlot& Q = E->h_slots[si];
+  memset(&Q, 0, sizeof(Q));
+  Q.state = QS_ACTIVE;
+  Q.l = hs.pq.l;
+  Q.h = hs.pq.h;
+  Q.limit_doms = hs.pq.ld;
+  const scs_params& p = E->p;
+  Q.flags = (p.red1 ? QF_RED1 : 0) | (p.red2 ? QF_RED2 : 0) | (p.red3 ? QF_RED3 : 0) |
+            (p.prun1 ? QF_PRUN1 : 0) | (p.prun2 ? QF_PRUN2 : 0) | QF_DOMS;
+  Q.ku = hs.ku;
+  Q.K = K;
+  Q.dmax = hg.dmax;
+  Q.qloc = qloc;
+  Q.kl = kl0;
+  Q.pending = 1;
+  Q.node_budget = p.node_budget;
+  Q.off = loff;
+  Q.adj = ladj;
+  Q.dist = ldist;
+  Q.orig = lorig;
+  Q.core = lcore;
+  Q.witness = E->d_witness + (u64)si * SCS_MAX_H;
+  return SCS_OK;
+}
+
+static void release_slot(scs_engine* E, u32 si) {
+  HostSlot& hs = E->slots[si];
+  if (hs.a_len) E->arena.release(hs.a_off, hs.a_len);
+  if (hs.b_len) E->arena.release(hs.b_off, hs.b_len);
+  hs.a_len = hs.b_len = 0;
+  hs.used = false;
+}
+
+static int collect(scs_engine* E) {
+  const u32 nq = E->p.limit_queries;
+  std::vector<QuerySlot> dev(nq);
+  HIPCHK(hipMemcpy(dev.data(), E->d_slots, sizeof(QuerySlot) * nq, hipMemcpyDeviceToHost));
+  std::vector<u32> wit(SCS_MAX_H);
+  for (u32 si = 0; si < nq; si++) {
+    HostSlot& hs = E->slots[si];
+    if (!hs.used || dev[si].state != QS_DONE) continue;
+    const QuerySlot& Q = dev[si];
+    u32 ws = std::min<u32>(Q.wsize, SCS_MAX_H);
+    if (ws)
+      HIPCHK(hipMemcpy(wit.data(), E->d_witness + (u64)si * SCS_MAX_H, sizeof(u32) * ws,
+                       hipMemcpyDeviceToHost));
+    int status;
+    if (Q.capped)
+      status = SCS_ST_CAPPED;
+    else if (Q.kl == 0 && hs.pq.l >= 2)
+      status = SCS_ST_INFEASIBLE;
+    else
+      status = SCS_ST_OK;
+    push_result(E, hs.pq, Q.kl, hs.kl0, hs.ku, status, hs.K, Q.nodes, wit.data(), ws);
+    E->st.nodes += Q.nodes;
+    release_slot(E, si);
+    // mark free on the device
+    u32 fr = QS_FREE;
+    HIPCHK(hipMemcpy(&E->d_slots[si].state, &fr, sizeof(u32), hipMemcpyHostToDevice));
+    E->active--;
+  }
+  return SCS_OK;
+}
+
+extern "C" int scs_step(scs_engine* E) {
+  if (!E) return SCS_EINVAL;
+  const scs::HostGraph& hg = E->g->hg;
+  // 1. take queued queries for free slots
+  std::vector<u32> free_slots;
+  for (u32 si = 0; si < E->p.limit_queries; si++)
+    if (!E->slots[si].used) free_slots.push_back(si);
+  std::vector<PendingQuery> batch;
+  {
+    std::lock_guard<std::mutex> lk(E->qmu);
+    while (!E->queue.empty() && batch.size() < free_slots.size()) {
+      batch.push_back(E->queue.front());
+      E->queue.pop_front();
+    }
+  }
+  // 2. heuristic lower bounds on host threads
+  std::vector<u32> kl0(batch.size(), 0);
+  std::vector<std::vector<u32>> H(batch.size());
+  {
+    double t0 = now_us();
+    std::atomic<size_t> next{0};
+    auto worker = [&](u32 t) {
+      size_t i;
+      while ((i = next.fetch_add(1)) < batch.size()) {
+        const PendingQuery& pq = batch[i];
+        if (pq.heu && pq.l <= pq.h) kl0[i] = scs::cssc_heu(hg, pq.l, pq.h, pq.q, E->heu_work[t], H[i]);
+      }
+    };
+    u32 nt = std::min<u32>(E->p.heuristic_threads, (u32)std::max<size_t>(batch.size(), 1));
+    std::vector<std::thread> th;
+    for (u32 t = 1; t < nt; t++) th.emplace_back(worker, t);
+    worker(0);
+    for (auto& t : th) t.join();
+    E->st.heu_us += now_us() - t0;
+  }
+  // 3. admission: immediate answers or GPU preprocessing
+  std::vector<u32> admitted;
+  size_t fs = 0;
+  for (size_t i = 0; i < batch.size(); i++) {
+    const PendingQuery& pq = batch[i];
+    const u32 ku = std::min(hg.core[pq.q], pq.h - 1);
+    if (pq.l > pq.h) {
+      push_result(E, pq, 0, 0, ku, SCS_ST_INFEASIBLE, 0, 0, nullptr, 0);
+      continue;
+    }
+    if (pq.heu && kl0[i] == ku) {
+      push_result(E, pq, kl0[i], kl0[i], ku, SCS_ST_HEU_OPT, 0, 0, H[i].data(), (u32)H[i].size());
+      continue;
+    }
+    const u32 si = free_slots[fs];
+    HostSlot& hs = E->slots[si];
+    hs.used = true;
+    hs.pq = pq;
+    hs.kl0 = kl0[i];
+    hs.ku = ku;
+    hs.K = 0;
+    int rc = prep_query(E, si, kl0[i], scs::ubd_host(kl0[i], pq.h));
+    if (rc == SCS_EBIG) {
+      push_result(E, pq, kl0[i], kl0[i], ku, SCS_ST_REJECTED, hs.K, 0, H[i].data(), (u32)H[i].size());
+      release_slot(E, si);
+      continue;
+    }
+    if (rc == SCS_EAGAIN) {
+      release_slot(E, si);
+      std::lock_guard<std::mutex> lk(E->qmu);
+      E->queue.push_front(pq);
+      continue;
+    }
+    if (rc) return rc;
+    // heuristic witness seeds the slot (only trusted when kl0 > 0)
+    QuerySlot& Q = E->h_slots[si];
+    if (kl0[i] > 0 && !H[i].empty() && H[i].size() <= SCS_MAX_H) {
+      Q.wsize = (u32)H[i].size();
+      Q.wkl = kl0[i];
+      HIPCHK(hipMemcpyAsync(Q.witness, H[i].data(), sizeof(u32) * H[i].size(),
+                            hipMemcpyHostToDevice, E->s));
+    }
+    HIPCHK(hipMemcpyAsync(&E->d_slots[si], &Q, sizeof(QuerySlot), hipMemcpyHostToDevice, E->s));
+    admitted.push_back(si);
+    fs++;
+  }
+  if (!admitted.empty()) {
+    HIPCHK(hipMemcpyAsync(E->d_admit, admitted.data(), sizeof(u32) * admitted.size(),
+                          hipMemcpyHostToDevice, E->s));
+    HIPCHK(scs_launch_admit(&E->args, E->d_admit, (u32)admitted.size(), E->d_admit + E->p.limit_queries,
+                            E->s));
+    std::vector<u32> failed(admitted.size());
+    HIPCHK(hipMemcpyAsync(failed.data(), E->d_admit + E->p.limit_queries,
+                          sizeof(u32) * admitted.size(), hipMemcpyDeviceToHost, E->s));
+    HIPCHK(hipStreamSynchronize(E->s));
+    for (u32 f : failed)
+      if (f) {
+        fprintf(stderr, "scs: task pool full at admission\n");
+        return SCS_ENOMEM;
+      }
+    E->active += (u32)admitted.size();
+  }
+  // 4. one search epoch
+  if (E->active > 0) {
+    EngineGlobals gl;
+    HIPCHK(hipMemcpy(&gl, E->d_glob, sizeof(gl), hipMemcpyDeviceToHost));
+    gl.idle = 0;
+    gl.active_q = E->active;
+    HIPCHK(hipMemcpy(E->d_glob, &gl, sizeof(gl), hipMemcpyHostToDevice));
+    HIPCHK(hipEventRecord(E->ev0, E->s));
+    HIPCHK(scs_launch_epoch(&E->args, E->s));
+    HIPCHK(hipEventRecord(E->ev1, E->s));
+    HIPCHK(hipEventSynchronize(E->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+    E->st.engine_us += ms * 1000.0;
+    E->st.epochs++;
+    int rc = collect(E);
+    if (rc) return rc;
+  }
+  size_t queued;
+  {
+    std::lock_guard<std::mutex> lk(E->qmu);
+    queued = E->queue.size();
+  }
+  return (int)(queued + E->active);
+}
+
+extern "C" int scs_poll(scs_engine* E, scs_result* out, int max) {
+  if (!E || (max > 0 && !out)) return SCS_EINVAL;
+  int c = 0;
+  while (c < max && !E->results.empty()) {
+    out[c++] = E->results.front();
+    E->results.erase(E->results.begin());
+  }
+  return c;
+}
+
+extern "C" int scs_drain(scs_engine* E) {
+  while (true) {
+    int r = scs_step(E);
+    if (r <= 0) return r;
+  }
+}
+
+extern "C" int scs_engine_stats(const scs_engine* E, scs_stats* s) {
+  if (!E || !s) return SCS_EINVAL;
+  *s = E->st;
+  EngineGlobals gl;
+  HIPCHK(hipMemcpy(&gl, E->d_glob, sizeof(gl), hipMemcpyDeviceToHost));
+  s->donations = gl.donations;
+  s->rebuilds = gl.rebuilds;
+  unsigned long long pb = 0;
+  HIPCHK(hipMemcpy(&pb, E->d_prepbytes, sizeof(pb), hipMemcpyDeviceToHost));
+  s->prep_bytes = pb;
+  return SCS_OK;
+}
+
+extern "C" const char* scs_strerror(int code) {
+  switch (code) {
+    case SCS_OK: return "ok";
+    case SCS_EINVAL: return "invalid argument";
+    case SCS_ENOMEM: return "out of memory";
+    case SCS_EAGAIN: return "admission refused for now";
+    case SCS_EHIP: return "HIP runtime error";
+    case SCS_EIO: return "graph file unreadable";
+    case SCS_EBIG: return "query exceeds per-query limits";
+    default: return "unknown error";
+  }
+}

@@ -1,0 +1,129 @@
+// Internal layouts shared by the host driver and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/scs.h"
+
+typedef uint8_t u8;
+typedef uint16_t u16;
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef int64_t i64;
+
+#define SCS_WAVE 64
+#define SCS_FRAME_DOMS SCS_MAX_DOMS
+#define SCS_HCAP (SCS_MAX_H + 2)
+
+// vertex status inside a search state
+#define ST_R 0
+#define ST_C 1
+#define ST_X 2
+
+// query slot states
+#define QS_FREE 0
+#define QS_ACTIVE 1
+#define QS_DONE 2
+
+// flags of a query (bit layout shared with the CPU oracle)
+#define QF_RED1 1u
+#define QF_RED2 2u
+#define QF_RED3 4u
+#define QF_PRUN1 8u
+#define QF_PRUN2 16u
+#define QF_DOMS 32u
+
+// Per-query slot, device resident.  Written by the host at admission (while
+// no kernel touches the slot) and by search waves through atomics.
+struct QuerySlot {
+  u32 state;       // QS_*
+  u32 l, h, limit_doms;
+  u32 flags;       // QF_*
+  u32 ku, K, dmax, qloc;
+  u32 kl;          // best min degree so far (atomicMax)
+  u32 stop;        // 1 = abandon remaining work (kl >= ku or budget)
+  u32 pending;     // tasks queued or being searched
+  u32 capped;      // node budget hit
+  u32 wlock;       // witness spin lock
+  u32 wkl;         // min degree of the stored witness
+  u32 wsize;       // |witness| (0 = none)
+  u64 nodes;       // nodes expanded (atomicAdd)
+  u64 node_budget; // 0 = unlimited
+  const u32* off;  // local CSR offsets (K+1)
+  const u32* adj;  // local CSR adjacency (local ids, ascending)
+  const u32* dist; // BFS distance from q in the core > kl0 subgraph
+  const u32* orig; // local -> original vertex id
+  const u32* core; // core numbers (MLP feature)
+  u32* witness;    // h entries, original ids
+  u32 pad[2];
+};
+
+// Depth-first frame: one per branching node on the current path.
+struct Frame {
+  u32 mark;    // trail length after the node's own rule changes
+  u32 ustar;
+  u32 nD;      // kept dominated vertices
+  u32 next;    // child being explored: 0 = A, i = B_{i-1}
+  u32 kl;      // kl at the end of the node's rule closure
+  u32 csize;   // |C| at the node
+  u32 zgone;   // 1 = Z child donated to the pool
+  u32 pad;
+  u32 D[SCS_FRAME_DOMS];
+};
+
+// Header of a wave's persistent depth-first state.
+struct WaveHdr {
+  u32 active;    // a task is being searched
+  u32 qslot;
+  u32 nframes;
+  u32 trail;     // trail length
+  u32 nC, nR;
+  u32 from;      // trail position where the pending node's delta starts
+  u32 kl_seen;   // kl at the parent's closure (0xffffffff = rescan all)
+  u32 K;         // kept-set size of the bound query
+  u32 pend_nodes;// nodes not yet added to the query counter
+  u32 pad[6];
+};
+
+// Offsets (bytes) of each per-wave array inside one wave's region.
+struct WaveLayout {
+  u64 st, nmark, queued, dC, dR, pos, rlist, acc, work, front, trail, frames, clist, hist, hdr;
+  u64 stride;
+};
+
+// Pool of donated tasks: bounded MPMC ring (per-slot sequence numbers).
+// Task payload: [qslot, nC, nL, form(0 = list is X, 1 = list is R)], C[nC], L[nL].
+struct Pool {
+  u64* seq;        // nslots
+  u32* data;       // nslots * slot_words
+  u64* enq;        // enqueue position
+  u64* deq;        // dequeue position
+  u32 nslots;
+  u32 slot_words;
+};
+
+struct EngineGlobals {
+  u32 idle;        // waves currently without work
+  u32 active_q;    // queries with pending work
+  u64 donations;
+  u64 rebuilds;
+  u64 nodes;
+  u32 lock_fail;
+  u32 pad[7];
+};
+
+struct EngineArgs {
+  QuerySlot* slots;
+  u32 nslots;
+  u8* wave_mem;
+  WaveLayout lay;
+  u32 kcap;
+  u32 nwaves;
+  Pool pool;
+  EngineGlobals* glob;
+  u64 slice_ticks;     // wall_clock64 ticks per epoch
+  u32 donate_below;    // donate while fewer tasks are pooled
+  u32 deterministic;
+  u32 node_flush;      // flush node counters every this many nodes
+  u32 pad;
+};

@@ -1,0 +1,257 @@
+"""Python host API over the C ABI in include/scs.h (ctypes).
+
+Mirrors the reference's process surface (main.cu): a graph loaded from an
+edge list or a CSR, an engine configured by the reference's twelve
+command-line parameters plus named extensions, queries submitted as
+``<l> <h> <q> <heuristic_flag> <limit_dominating_set>`` and results with the
+reference's columns (N1, N2, QID, time, degree, overtime, heu).
+
+The library must be built (scs_amd/build.py); nothing here falls back to a
+CPU path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "libscs.so")
+_LIB = None
+
+MAX_H = 1000
+MAX_DOMS = 32
+STATUS = {0: "OK", 1: "HEU_OPT", 2: "CAPPED", 3: "INFEASIBLE", 4: "REJECTED"}
+ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EAGAIN", -4: "EHIP", -5: "EIO", -6: "EBIG"}
+
+
+class SCSError(RuntimeError):
+    def __init__(self, code, what=""):
+        super().__init__(f"{what}: {ERRORS.get(code, code)}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("partition_size", C.c_uint32), ("buffer_size", C.c_uint32),
+                ("copy_limit", C.c_double), ("read_limit", C.c_uint32),
+                ("limit_queries", C.c_uint32), ("factor", C.c_uint32),
+                ("red1", C.c_uint32), ("red2", C.c_uint32), ("red3", C.c_uint32),
+                ("prun1", C.c_uint32), ("prun2", C.c_uint32),
+                ("waves_per_cu", C.c_uint32), ("limit_tasks", C.c_uint32),
+                ("node_budget", C.c_uint64), ("mem_budget", C.c_uint64),
+                ("max_kept", C.c_uint32), ("time_slice_us", C.c_uint32),
+                ("deterministic", C.c_uint32), ("learned", C.c_uint32),
+                ("model", C.c_void_p), ("heuristic_threads", C.c_uint32),
+                ("reserved", C.c_uint32 * 7)]
+
+
+class Result(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("l", C.c_uint32), ("h", C.c_uint32), ("q", C.c_uint32),
+                ("kl", C.c_uint32), ("heu_kl", C.c_uint32), ("ku", C.c_uint32),
+                ("status", C.c_int32), ("kept", C.c_uint32), ("nodes", C.c_uint64),
+                ("t_submit_us", C.c_double), ("t_done_us", C.c_double),
+                ("wsize", C.c_uint32), ("witness", C.c_uint32 * MAX_H)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("nodes", C.c_uint64), ("donations", C.c_uint64), ("rebuilds", C.c_uint64),
+                ("engine_us", C.c_double), ("prep_us", C.c_double), ("heu_us", C.c_double),
+                ("prep_bytes", C.c_uint64), ("epochs", C.c_uint64), ("waves", C.c_uint32),
+                ("kcap", C.c_uint32)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_LIBPATH):
+            raise RuntimeError(f"{_LIBPATH} is missing: run scs_amd/build.py (no CPU fallback)")
+        L = C.CDLL(_LIBPATH)
+        P = C.c_void_p
+        L.scs_graph_create.argtypes = [np.ctypeslib.ndpointer(np.int64, flags="C"),
+                                       np.ctypeslib.ndpointer(np.int32, flags="C"),
+                                       C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(P)]
+        L.scs_graph_load.argtypes = [C.c_char_p, C.POINTER(P)]
+        L.scs_graph_info.argtypes = [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_uint32)]
+        L.scs_graph_core.argtypes = [P, np.ctypeslib.ndpointer(np.uint32, flags="C")]
+        L.scs_graph_csr.argtypes = [P, np.ctypeslib.ndpointer(np.int64, flags="C"),
+                                    np.ctypeslib.ndpointer(np.int32, flags="C")]
+        L.scs_graph_core_time_us.restype = C.c_double
+        L.scs_graph_core_time_us.argtypes = [P]
+        L.scs_graph_destroy.argtypes = [P]
+        L.scs_params_default.argtypes = [C.POINTER(Params)]
+        L.scs_engine_create.argtypes = [P, C.POINTER(Params), C.POINTER(P)]
+        L.scs_submit.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.c_uint64]
+        L.scs_submit_text.argtypes = [P, C.c_char_p, C.c_uint64]
+        L.scs_step.argtypes = [P]
+        L.scs_poll.argtypes = [P, C.POINTER(Result), C.c_int]
+        L.scs_drain.argtypes = [P]
+        L.scs_engine_destroy.argtypes = [P]
+        L.scs_engine_stats.argtypes = [P, C.POINTER(Stats)]
+        L.scs_strerror.restype = C.c_char_p
+        L.scs_strerror.argtypes = [C.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise SCSError(rc, what)
+    return rc
+
+
+class Graph:
+    """A graph resident on the GPU with its core numbers (computed there)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        n, nnz, dmax = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        _check(lib().scs_graph_info(self._h, C.byref(n), C.byref(nnz), C.byref(dmax)), "info")
+        self.n, self.nnz, self.dmax = n.value, nnz.value, dmax.value
+
+    @classmethod
+    def from_csr(cls, off, adj):
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        adj = np.ascontiguousarray(adj, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib().scs_graph_create(off, adj, len(off) - 1, adj.size, 0, C.byref(h)),
+               "scs_graph_create")
+        return cls(h)
+
+    @classmethod
+    def load(cls, path):
+        h = C.c_void_p()
+        _check(lib().scs_graph_load(path.encode(), C.byref(h)), f"scs_graph_load({path})")
+        return cls(h)
+
+    def core(self):
+        c = np.zeros(self.n, np.uint32)
+        _check(lib().scs_graph_core(self._h, c), "core")
+        return c
+
+    def csr(self):
+        off = np.zeros(self.n + 1, np.int64)
+        adj = np.zeros(max(self.nnz, 1), np.int32)
+        _check(lib().scs_graph_csr(self._h, off, adj), "csr")
+        return off, adj[:self.nnz]
+
+    @property
+    def core_time_us(self):
+        return lib().scs_graph_core_time_us(self._h)
+
+    def close(self):
+        if self._h:
+            lib().scs_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class QueryResult:
+    tag: int
+    l: int
+    h: int
+    q: int
+    kl: int
+    heu_kl: int
+    ku: int
+    status: str
+    kept: int
+    nodes: int
+    latency_us: float
+    witness: np.ndarray
+
+    def line(self):
+        """Reference results-file row N1|N2|QID|Time|Degree|Overtime|Heu|0
+        (main.cu:123-128, 328-333); Overtime 1 = node budget reached."""
+        over = 1 if self.status == "CAPPED" else 0
+        heu = 1 if self.status == "HEU_OPT" else 0
+        return f"{self.l}|{self.h}|{self.q}|{_commas(int(self.latency_us))}|{self.kl}|{over}|{heu}|0"
+
+
+def _commas(x: int) -> str:
+    return f"{x:,}"
+
+
+class Engine:
+    """Search engine over one graph (see scs_params in include/scs.h)."""
+
+    def __init__(self, graph: Graph, **kw):
+        p = Params()
+        lib().scs_params_default(C.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise TypeError(f"unknown engine parameter {k}")
+            setattr(p, k, v)
+        self.graph = graph
+        self.params = p
+        h = C.c_void_p()
+        _check(lib().scs_engine_create(graph._h, C.byref(p), C.byref(h)), "scs_engine_create")
+        self._h = h
+        self._buf = (Result * 256)()
+
+    def submit(self, l, h, q, heu=1, limit_doms=8, tag=0):
+        return _check(lib().scs_submit(self._h, l, h, q, heu, limit_doms, tag), "scs_submit")
+
+    def submit_text(self, text, tag=0):
+        return _check(lib().scs_submit_text(self._h, text.encode(), tag), "scs_submit_text")
+
+    def step(self):
+        return _check(lib().scs_step(self._h), "scs_step")
+
+    def poll(self):
+        out = []
+        while True:
+            k = _check(lib().scs_poll(self._h, self._buf, 256), "scs_poll")
+            for i in range(k):
+                r = self._buf[i]
+                out.append(QueryResult(
+                    tag=r.tag, l=r.l, h=r.h, q=r.q, kl=r.kl, heu_kl=r.heu_kl, ku=r.ku,
+                    status=STATUS.get(r.status, str(r.status)), kept=r.kept, nodes=r.nodes,
+                    latency_us=r.t_done_us - r.t_submit_us,
+                    witness=np.ctypeslib.as_array(r.witness)[:r.wsize].copy()))
+            if k < 256:
+                return out
+
+    def drain(self):
+        return _check(lib().scs_drain(self._h), "scs_drain")
+
+    def stats(self):
+        s = Stats()
+        _check(lib().scs_engine_stats(self._h, C.byref(s)), "stats")
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().scs_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, queries, tags=None):
+        """Submits all queries, drains the engine and returns results by tag."""
+        for i, qq in enumerate(queries):
+            l, h, q, heu, ld = qq
+            self.submit(l, h, q, heu, ld, i if tags is None else tags[i])
+        res = []
+        while True:
+            left = self.step()
+            res.extend(self.poll())
+            if left == 0:
+                break
+        res.extend(self.poll())
+        res.sort(key=lambda r: r.tag)
+        return res
