@@ -80,7 +80,7 @@ typedef struct scs_params {
   uint32_t red1, red2, red3;  /* argv[8..10]: reduction rules 1-3 */
   uint32_t prun1, prun2;      /* argv[11..12]: bounds U_d and U_nr */
   /* named extensions */
-  uint32_t waves_per_cu;   /* resident search waves per CU (0 = 4) */
+  uint32_t waves_per_cu;   /* resident search waves per CU (0 = 16, max 16) */
   uint32_t limit_tasks;    /* task pool slots (0 = 4 * waves) */
   uint64_t node_budget;    /* per-query node cap before CAPPED (0 = none) */
   uint64_t mem_budget;     /* bytes of device memory for per-query graphs */
@@ -156,8 +156,26 @@ typedef struct scs_stats {
   uint64_t epochs;
   uint32_t waves;
   uint32_t kcap;
+  uint64_t edges;          /* adjacency entries scanned by the search */
+  double core_us;          /* device time of the core decomposition */
 } scs_stats;
 int scs_engine_stats(const scs_engine* e, scs_stats* s);
+
+/* Host-only: the greedy lower bound of the reference's CSSC_heu
+ * (inc/heuristic.h:321-348) on a CSR; writes kl and H (up to cap ids). */
+int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_t n, uint32_t l,
+                      uint32_t h, uint32_t q, uint32_t* kl, uint32_t* H, uint32_t cap,
+                      uint32_t* hsize);
+
+/* Runs the GPU per-query preprocessing (bounded BFS, compaction, local CSR)
+ * for (q, kl0, ubD) and copies the result out: K kept vertices (original
+ * ids, ascending), local offsets (K+1), adjacency (local ids) and BFS
+ * distances.  *K and *nnz are always set; arrays are written only when
+ * they fit (kept_cap, adj_cap).  Replaces the reference's per-query
+ * initialReductionRules/CompressTask/NeighborUpdate (src/helpers.cc:207-368). */
+int scs_prep_probe(scs_engine* e, uint32_t q, uint32_t kl0, uint32_t ubD, uint32_t* K,
+                   uint64_t* nnz, uint32_t* kept, uint32_t* loff, uint32_t* dist,
+                   uint32_t kept_cap, uint32_t* ladj, uint64_t adj_cap);
 
 const char* scs_strerror(int code);
 

@@ -62,6 +62,8 @@ def lib():
         L.orc_ubd_host.argtypes = [C.c_uint32, C.c_uint32]
         L.orc_ubd_node.restype = C.c_uint32
         L.orc_ubd_node.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_unr_bound.restype = C.c_uint32
+        L.orc_unr_bound.argtypes = [u32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32]
         L.orc_heuristic.restype = C.c_uint32
         L.orc_heuristic.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p,
                                     C.c_uint32, C.POINTER(C.c_uint32)]
@@ -163,6 +165,15 @@ class OracleGraph:
 
 def ubd_host(kl, h):
     return int(lib().orc_ubd_host(kl, h))
+
+
+def unr_bound(cvals, rvals, k):
+    c = np.ascontiguousarray(cvals, np.uint32)
+    r = np.ascontiguousarray(rvals, np.uint32)
+    if r.size == 0:
+        r = np.zeros(1, np.uint32)
+        return int(lib().orc_unr_bound(c, c.size, r, 0, k))
+    return int(lib().orc_unr_bound(c, c.size, r, r.size, k))
 
 
 def ubd_node(kl, h):

@@ -275,6 +275,10 @@ static void heu4(const Graph& g, u32 N1, u32 N2, u32 q, std::vector<u32>& H, u32
     Q.pop();
     if (!inS[v]) continue;
     u32 remain = (u32)S.size() - (u32)rmv.size();
+    // q carries a +1 (inc/heuristic.h:259) that keeps it from being peeled
+    // before the others; it is popped only when alone, where its true
+    // degree is 0, so the +1 is taken back for the min-degree test
+    if (v == q) d -= 1;
     if (remain >= N1 && remain <= N2 && d > mindeg) {
       mindeg = d;
       rmv_idx = (u32)rmv.size();
@@ -1113,6 +1117,16 @@ void orc_bfs(orc_graph* o, u32 q, u32* dist) {
   std::copy(d.begin(), d.end(), dist);
 }
 u32 orc_ubd_host(u32 kl, u32 h) { return ubd_host(kl, h); }
+// U_nr water-filling on explicit values (for tests): cvals = dC over C,
+// rvals = dC over R, k rounds.
+u32 orc_unr_bound(const u32* cvals, u32 m, const u32* rvals, u32 nr, u32 k) {
+  std::vector<u32> cv(cvals, cvals + m);
+  u32 mx = 0;
+  for (u32 i = 0; i < nr; i++) mx = std::max(mx, rvals[i]);
+  std::vector<u32> rh(mx + 2, 0);
+  for (u32 i = 0; i < nr; i++) rh[rvals[i]]++;
+  return unr_bound(cv, rh, k);
+}
 u32 orc_ubd_node(u32 kl, u32 h) { return ubd_node(kl, h); }
 
 // heuristic: which = 2, 3, 4 or 0 (CSSC_heu).  Writes up to cap witness
@@ -1185,6 +1199,12 @@ int orc_solve(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 fl
   if (heu && kl == ku) {
     res->kl = kl;
     res->status = 1;
+    if (kl == 0) {
+      // a connected H of two or more vertices has min degree >= 1
+      res->status = l >= 2 ? 3 : 1;
+      H.assign(1, q);
+      if (l >= 2) H.clear();
+    }
     res->wsize = (u32)H.size();
     for (u32 i = 0; i < H.size() && i < wcap; i++) wit[i] = H[i];
     return 0;
@@ -1231,6 +1251,10 @@ int orc_solve(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 fl
   if (!wloc.empty()) {
     H.clear();
     for (u32 v : wloc) H.push_back(L.orig[v]);
+  }
+  if (final_kl == 0) {
+    H.clear();
+    if (l <= 1) H.push_back(q);
   }
   res->wsize = (u32)H.size();
   for (u32 i = 0; i < H.size() && i < wcap; i++) wit[i] = H[i];

@@ -364,7 +364,7 @@ extern "C" void scs_params_default(scs_params* p) {
   p->factor = 1;
   p->red1 = p->red2 = p->red3 = 1;
   p->prun1 = p->prun2 = 1;
-  p->waves_per_cu = 4;
+  p->waves_per_cu = 16;
   p->limit_tasks = 0;
   p->node_budget = 0;
   p->mem_budget = 1ull << 30;
@@ -391,11 +391,15 @@ static WaveLayout make_layout(u32 kcap) {
   L.rlist = o; o = align256(o + 4 * k4);
   L.acc = o; o = align256(o + 8 * k4);
   L.work = o; o = align256(o + 4 * (k4 + 64));
+  L.work2 = o; o = align256(o + 4 * (k4 + 64));
   L.front = o; o = align256(o + 4 * (k4 + 64));
-  L.trail = o; o = align256(o + 4 * (k4 + 64));
+  L.aux = o; o = align256(o + 4 * (k4 + 64));
+  // each vertex changes once per path; each removal batch adds one marker
+  L.trail = o; o = align256(o + 4 * (2 * k4 + 64));
+  L.sav = o; o = align256(o + 4 * (2 * k4 + 64));
   L.frames = o; o = align256(o + sizeof(Frame) * (SCS_HCAP + 1));
   L.clist = o; o = align256(o + 4 * (SCS_HCAP + 64));
-  L.hist = o; o = align256(o + 4 * (SCS_HCAP + 4));
+  L.hist = o; o = align256(o + 4 * (SCS_HCAP + 8));
   L.hdr = o; o = align256(o + sizeof(WaveHdr));
   L.stride = o;
   return L;
@@ -409,7 +413,8 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   else
     scs_params_default(&p);
   if (p.limit_queries == 0) p.limit_queries = 1;
-  if (p.waves_per_cu == 0) p.waves_per_cu = 4;
+  if (p.waves_per_cu == 0) p.waves_per_cu = 16;
+  if (p.waves_per_cu > 16) p.waves_per_cu = 16;  // LDS per wave (~8 KB) x 16 fits 160 KB
   if (p.time_slice_us == 0) p.time_slice_us = 2000;
   if (p.heuristic_threads == 0) p.heuristic_threads = 4;
   scs_engine* E = new scs_engine;
@@ -685,6 +690,11 @@ static int collect(scs_engine* E) {
       status = SCS_ST_INFEASIBLE;
     else
       status = SCS_ST_OK;
+    if (Q.kl == 0) {
+      // min degree 0 is only feasible as H = {q}
+      ws = hs.pq.l <= 1 ? 1 : 0;
+      wit[0] = hs.pq.q;
+    }
     push_result(E, hs.pq, Q.kl, hs.kl0, hs.ku, status, hs.K, Q.nodes, wit.data(), ws);
     E->st.nodes += Q.nodes;
     release_slot(E, si);
@@ -742,6 +752,15 @@ extern "C" int scs_step(scs_engine* E) {
       continue;
     }
     if (pq.heu && kl0[i] == ku) {
+      if (ku == 0) {
+        // a connected H of two or more vertices has min degree >= 1
+        const u32 one = pq.q;
+        if (pq.l >= 2)
+          push_result(E, pq, 0, 0, ku, SCS_ST_INFEASIBLE, 0, 0, nullptr, 0);
+        else
+          push_result(E, pq, 0, 0, ku, SCS_ST_HEU_OPT, 0, 0, &one, 1);
+        continue;
+      }
       push_result(E, pq, kl0[i], kl0[i], ku, SCS_ST_HEU_OPT, 0, 0, H[i].data(), (u32)H[i].size());
       continue;
     }
@@ -843,10 +862,63 @@ extern "C" int scs_engine_stats(const scs_engine* E, scs_stats* s) {
   HIPCHK(hipMemcpy(&gl, E->d_glob, sizeof(gl), hipMemcpyDeviceToHost));
   s->donations = gl.donations;
   s->rebuilds = gl.rebuilds;
+  s->edges = gl.edges;
+  s->core_us = E->g->core_us;
   unsigned long long pb = 0;
   HIPCHK(hipMemcpy(&pb, E->d_prepbytes, sizeof(pb), hipMemcpyDeviceToHost));
   s->prep_bytes = pb;
   return SCS_OK;
+}
+
+extern "C" int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_t n, uint32_t l,
+                                 uint32_t h, uint32_t q, uint32_t* kl, uint32_t* H, uint32_t cap,
+                                 uint32_t* hsize) {
+  if (!off || !kl || q >= n || h == 0) return SCS_EINVAL;
+  scs::HostGraph g;
+  g.n = n;
+  g.off.assign(off, off + n + 1);
+  g.adj.assign(adj, adj + off[n]);
+  g.dmax = 0;
+  for (u32 v = 0; v < n; v++) g.dmax = std::max<u32>(g.dmax, (u32)(off[v + 1] - off[v]));
+  scs::HeuWork W;
+  std::vector<u32> HH;
+  *kl = scs::cssc_heu(g, l, h, q, W, HH);
+  if (hsize) *hsize = (u32)HH.size();
+  if (H)
+    for (u32 i = 0; i < HH.size() && i < cap; i++) H[i] = HH[i];
+  return SCS_OK;
+}
+
+extern "C" int scs_prep_probe(scs_engine* E, uint32_t q, uint32_t kl0, uint32_t ubD, uint32_t* K,
+                              uint64_t* nnz, uint32_t* kept, uint32_t* loff, uint32_t* dist,
+                              uint32_t kept_cap, uint32_t* ladj, uint64_t adj_cap) {
+  if (!E || !K || !nnz || q >= E->g->hg.n) return SCS_EINVAL;
+  // use a free slot's bookkeeping temporarily
+  u32 si = 0;
+  while (si < E->p.limit_queries && E->slots[si].used) si++;
+  if (si == E->p.limit_queries) return SCS_EAGAIN;
+  HostSlot& hs = E->slots[si];
+  hs.pq = PendingQuery{0, 1, 1, q, 0, 0, now_us()};
+  hs.ku = 0;
+  int rc = prep_query(E, si, kl0, ubD);
+  *K = hs.K;
+  *nnz = 0;
+  if (rc == SCS_OK) {
+    const QuerySlot& Q = E->h_slots[si];
+    const u32 k = hs.K;
+    u32 nz = 0;
+    HIPCHK(hipMemcpy(&nz, Q.off + k, sizeof(u32), hipMemcpyDeviceToHost));
+    *nnz = nz;
+    if (k <= kept_cap) {
+      if (kept) HIPCHK(hipMemcpy(kept, Q.orig, sizeof(u32) * k, hipMemcpyDeviceToHost));
+      if (loff) HIPCHK(hipMemcpy(loff, Q.off, sizeof(u32) * (k + 1), hipMemcpyDeviceToHost));
+      if (dist) HIPCHK(hipMemcpy(dist, Q.dist, sizeof(u32) * k, hipMemcpyDeviceToHost));
+    }
+    if (ladj && nz <= adj_cap)
+      HIPCHK(hipMemcpy(ladj, Q.adj, sizeof(u32) * (u64)nz, hipMemcpyDeviceToHost));
+  }
+  release_slot(E, si);
+  return rc == SCS_EBIG ? SCS_OK : rc;
 }
 
 extern "C" const char* scs_strerror(int code) {

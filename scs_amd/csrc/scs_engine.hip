@@ -6,14 +6,18 @@
 // Design: one wavefront (64 lanes, one workgroup) is one depth-first
 // worker.  It keeps, for the query it is bound to, the full search state in
 // its own HBM region: vertex status (R = candidate, C = chosen, X = out),
-// dC = |N(v) & C|, dR = |N(v) & R|, an explicit list of R, a histogram of
-// dC over R and an undo trail.  A node costs O(changes x degree) instead of
-// the reference's O(|T| x deg x |T|) rescans (findIndexKernel,
-// src/helpers.cc:23-47).  Waves that run out of work take tasks from a
-// bounded lock-free pool that busy waves fill by donating the unexplored
-// "exclude u*" branch of their shallowest frame.  The kernel runs for a
-// bounded time slice and returns; all state persists in HBM between slices,
-// so the host never has to wait on a spinning kernel.
+// dC = |N(v) & C|, dR = |N(v) & R|, an explicit list of R, an undo trail,
+// and (in LDS) a histogram of dC over R.  A node costs O(changes x degree)
+// instead of the reference's O(|T| x deg x |T|) rescans (findIndexKernel,
+// src/helpers.cc:23-47).  Removals are applied in batches: every lane takes
+// removed vertices and decrements their neighbours' dR with atomics, and an
+// undo restores a whole batch the same way, so the large rule 1/2 closures
+// deep in the tree are parallel instead of vertex by vertex.  Waves that
+// run out of work take tasks from a bounded lock-free pool that busy waves
+// fill by donating the unexplored "exclude u*" branch of their shallowest
+// frame.  The kernel runs for a bounded time slice and returns; all state
+// persists in HBM between slices, so the host never waits on a spinning
+// kernel.
 //
 // Node semantics (identical to the CPU oracle, so a single wave expands
 // exactly the oracle's nodes): min degree of C when l <= |C| <= h, rule 3,
@@ -23,6 +27,13 @@
 
 #define LANE (threadIdx.x)
 #define INV 0xffffffffu
+// trail entries: v | TR_MOVE (R -> C), v (R -> X, part of a batch), and a
+// batch marker TR_MARK | m after the m vertices of a batch; v < 2^30
+#define TR_MOVE 0x80000000u
+#define TR_MARK 0x40000000u
+#define TR_RECOUNT 0x20000000u  // marker flag: batch applied by recounting
+#define TR_VMASK 0x3fffffffu
+#define TR_MMASK 0x1fffffffu
 
 // ---------------------------------------------------------------------------
 // Wave primitives (64 lanes; one wave per workgroup).
@@ -78,6 +89,14 @@ __device__ u32 ubd_node(u32 kl, u32 h) {
   return ub;
 }
 
+// LDS of one wave (one workgroup).
+struct Lds {
+  u32 cnt, cnt2, nfront, ncand, nh, nk, pad0, pad1;
+  u32 D[SCS_FRAME_DOMS];
+  u32 hist[SCS_HCAP + 4];   // dC histogram of R (live copy of the wave's state)
+  u32 chist[SCS_HCAP + 4];  // scratch histogram of C degrees (U_nr)
+};
+
 // ---------------------------------------------------------------------------
 // Per-wave state view.
 // ---------------------------------------------------------------------------
@@ -91,23 +110,28 @@ struct W {
   u32* rlist;
   u64* acc;
   u32* work;
+  u32* work2;
   u32* front;
+  u32* aux;
   u32* trail;
+  u32* sav;    // saved (vertex, dR) pairs of recount batches, as a stack
   Frame* frames;
   u32* clist;
-  u32* hist;   // h+2 counters of dC over R
+  u32* ghist;  // HBM copy of the histogram between epochs
   WaveHdr* hdr;
+  Lds* L;
   // query
   QuerySlot* Q;
   const u32* off;
   const u32* adj;
   const u32* dist;
   u32 K, l, h, flags, limit_doms, dmax, ku;
-  // cached header fields (wave-uniform)
-  u32 nC, nR, ntrail, nframes;
+  // wave-uniform counters
+  u32 nC, nR, ntrail, nframes, nsav;
+  u64 edges;
 };
 
-__device__ void bind(W& w, const EngineArgs& A, u32 wave) {
+__device__ void bind(W& w, const EngineArgs& A, u32 wave, Lds* L) {
   u8* base = A.wave_mem + (u64)wave * A.lay.stride;
   w.st = base + A.lay.st;
   w.nmark = base + A.lay.nmark;
@@ -118,12 +142,18 @@ __device__ void bind(W& w, const EngineArgs& A, u32 wave) {
   w.rlist = (u32*)(base + A.lay.rlist);
   w.acc = (u64*)(base + A.lay.acc);
   w.work = (u32*)(base + A.lay.work);
+  w.work2 = (u32*)(base + A.lay.work2);
   w.front = (u32*)(base + A.lay.front);
+  w.aux = (u32*)(base + A.lay.aux);
   w.trail = (u32*)(base + A.lay.trail);
+  w.sav = (u32*)(base + A.lay.sav);
   w.frames = (Frame*)(base + A.lay.frames);
   w.clist = (u32*)(base + A.lay.clist);
-  w.hist = (u32*)(base + A.lay.hist);
+  w.ghist = (u32*)(base + A.lay.hist);
   w.hdr = (WaveHdr*)(base + A.lay.hdr);
+  w.L = L;
+  w.edges = 0;
+  w.nsav = 0;
 }
 
 __device__ void bind_query(W& w, QuerySlot* Q) {
@@ -141,106 +171,253 @@ __device__ void bind_query(W& w, QuerySlot* Q) {
 }
 
 // ---------------------------------------------------------------------------
-// State updates.  Counters of X vertices are frozen; moves and removals only
-// touch neighbours that are still live, and undo applies the exact inverse
-// (the trail restores statuses in reverse order, so the same neighbours are
-// live again when an entry is undone).
+// State updates.  Counters of X vertices are frozen: a move or a removal
+// updates only the neighbours that are live at that moment, and an undo,
+// which runs in exact reverse order, finds the same neighbours live again.
+// A removal batch decrements first (all batch members still R) and then
+// marks the batch X; its undo marks the batch R first and then increments,
+// so batch members see each other both ways.
 // ---------------------------------------------------------------------------
-__device__ void rlist_remove(W& w, u32 v) {  // lane 0 only
-  u32 p = w.pos[v];
-  u32 last = w.rlist[w.nR - 1];
-  w.rlist[p] = last;
-  w.pos[last] = p;
-  w.nR--;
-}
-__device__ void rlist_add(W& w, u32 v) {  // lane 0 only
-  w.rlist[w.nR] = v;
-  w.pos[v] = w.nR;
-  w.nR++;
-}
-
 __device__ void move_c(W& w, u32 v) {
   const u32 b = w.off[v], e = w.off[v + 1];
+  u32* hist = w.L->hist;
   for (u32 j = b + LANE; j < e; j += 64) {
     u32 x = w.adj[j];
     u8 s = w.st[x];
     if (s == ST_X) continue;
     u32 dc = w.dC[x];
     if (s == ST_R) {
-      atomicSub(&w.hist[dc], 1u);
-      atomicAdd(&w.hist[dc + 1], 1u);
+      atomicSub(&hist[dc], 1u);
+      atomicAdd(&hist[dc + 1], 1u);
     }
     w.dC[x] = (u16)(dc + 1);
     w.dR[x] -= 1;
   }
+  w.edges += e - b;
   wsync();
   if (LANE == 0) {
     w.st[v] = ST_C;
-    w.hist[w.dC[v]] -= 1;
-    rlist_remove(w, v);
+    hist[w.dC[v]] -= 1;
+    u32 p = w.pos[v];
+    u32 last = w.rlist[w.nR - 1];
+    w.rlist[p] = last;
+    w.pos[last] = p;
     w.clist[w.nC] = v;
-    w.nC++;
-    w.trail[w.ntrail] = v | 0x80000000u;
-    w.ntrail++;
+    w.trail[w.ntrail] = v | TR_MOVE;
   }
-  w.nR = bcast(w.nR);
-  w.nC = bcast(w.nC);
-  w.ntrail = bcast(w.ntrail);
+  w.nR--;
+  w.nC++;
+  w.ntrail++;
   wsync();
 }
 
-__device__ void remove_r(W& w, u32 v) {
-  const u32 b = w.off[v], e = w.off[v + 1];
-  for (u32 j = b + LANE; j < e; j += 64) {
+__device__ void undo_move(W& w, u32 v) {
+  u32* hist = w.L->hist;
+  if (LANE == 0) {
+    w.st[v] = ST_R;
+    hist[w.dC[v]] += 1;
+    w.rlist[w.nR] = v;
+    w.pos[v] = w.nR;
+  }
+  w.nR++;
+  w.nC--;
+  wsync();
+  const u32 b = w.off[v], en = w.off[v + 1];
+  for (u32 j = b + LANE; j < en; j += 64) {
     u32 x = w.adj[j];
-    if (w.st[x] != ST_X) w.dR[x] -= 1;
+    u8 s = w.st[x];
+    if (s == ST_X) continue;
+    u32 dc = w.dC[x];
+    if (s == ST_R) {
+      atomicSub(&hist[dc], 1u);
+      atomicAdd(&hist[dc - 1], 1u);
+    }
+    w.dC[x] = (u16)(dc - 1);
+    w.dR[x] += 1;
+  }
+  w.edges += en - b;
+  wsync();
+}
+
+// Closure parameters for crossing detection during a batch (rule 1).
+struct Clo {
+  u32 on;   // push vertices whose dC + dR falls to kl
+  u32 kl;
+  u32* next;
+  u32* cnt;  // LDS counter of next
+};
+
+__device__ __forceinline__ void dec_neighbour(W& w, u32 x, const Clo& c) {
+  u8 s = w.st[x];
+  if (s == ST_X) return;
+  u32 old = atomicSub(&w.dR[x], 1u);
+  if (c.on && s == ST_R) {
+    u32 dc = w.dC[x];
+    if (dc <= c.kl && old + dc == c.kl + 1) {
+      u32* wp = (u32*)((uintptr_t)(w.queued + x) & ~(uintptr_t)3);
+      u32 sh = ((uintptr_t)(w.queued + x) & 3) * 8;
+      u32 o2 = atomicOr(wp, 1u << sh);
+      if (!((o2 >> sh) & 0xffu)) c.next[atomicAdd(c.cnt, 1u)] = x;
+    }
+  }
+}
+
+// Removes the m vertices of list (all currently R) as one trail batch.
+//
+// Two equivalent ways to apply it.  Exact mode decrements dR of every live
+// neighbour (cost: the batch's total degree).  Recount mode, used when the
+// batch is much larger than what stays live, marks the batch X without
+// touching its neighbours and then recounts dR of the surviving live
+// vertices (cost: their total degree), pushing the old values on the save
+// stack for the undo.  Both leave identical dC/dR on every live vertex.
+__device__ void remove_batch(W& w, const u32* list, u32 m, const Clo& c) {
+  if (m == 0) return;
+  u32* hist = w.L->hist;
+  const u32 live_after = w.nC + (w.nR - m);
+  const bool recount = m >= 256 && m > 2 * live_after &&
+                       (u64)w.nsav + 2ull * live_after + 1 <= 2ull * w.K + 64;
+  // phase 1 (exact mode): decrement the live neighbours (batch included)
+  if (!recount) {
+    if (m >= 8) {
+      u32 ed = 0;
+      for (u32 i = LANE; i < m; i += 64) {
+        u32 v = list[i];
+        const u32 b = w.off[v], e = w.off[v + 1];
+        ed += e - b;
+        for (u32 j = b; j < e; j++) dec_neighbour(w, w.adj[j], c);
+      }
+      w.edges += wsum(ed);
+    } else {
+      for (u32 i = 0; i < m; i++) {
+        u32 v = list[i];
+        const u32 b = w.off[v], e = w.off[v + 1];
+        for (u32 j = b + LANE; j < e; j += 64) dec_neighbour(w, w.adj[j], c);
+        w.edges += e - b;
+      }
+    }
+  }
+  if (LANE == 0) {
+    w.L->nh = 0;
+    w.L->nk = 0;
   }
   wsync();
-  if (LANE == 0) {
+  // phase 2: mark X, histogram, trail, queued flags
+  for (u32 i = LANE; i < m; i += 64) {
+    u32 v = list[i];
     w.st[v] = ST_X;
-    w.hist[w.dC[v]] -= 1;
-    rlist_remove(w, v);
-    w.trail[w.ntrail] = v;
-    w.ntrail++;
+    w.queued[v] = 0;
+    atomicSub(&hist[w.dC[v]], 1u);
+    w.trail[w.ntrail + i] = v;
   }
-  w.nR = bcast(w.nR);
-  w.ntrail = bcast(w.ntrail);
+  wsync();
+  // phase 3: fill the holes the batch leaves in R's list with the surviving
+  // entries of the list's last m positions
+  const u32 tail = w.nR - m;
+  for (u32 i = LANE; i < m; i += 64) {
+    u32 p = w.pos[list[i]];
+    if (p < tail) w.aux[atomicAdd(&w.L->nh, 1u)] = p;
+  }
+  for (u32 t = tail + LANE; t < w.nR; t += 64) {
+    u32 x = w.rlist[t];
+    if (w.st[x] == ST_R) w.front[atomicAdd(&w.L->nk, 1u)] = x;
+  }
+  wsync();
+  const u32 nh = bcast(w.L->nh);
+  for (u32 r = LANE; r < nh; r += 64) {
+    u32 p = w.aux[r], x = w.front[r];
+    w.rlist[p] = x;
+    w.pos[x] = p;
+  }
+  if (LANE == 0) w.trail[w.ntrail + m] = TR_MARK | (recount ? TR_RECOUNT : 0u) | m;
+  w.ntrail += m + 1;
+  w.nR = tail;
+  wsync();
+  if (recount) {
+    // phase 4: save and recount dR of the live vertices (R list, then C)
+    u32* S = w.sav + w.nsav;
+    u32 ed = 0;
+    for (u32 i = LANE; i < live_after; i += 64) {
+      const u32 v = i < w.nR ? w.rlist[i] : w.clist[i - w.nR];
+      const u32 b = w.off[v], e = w.off[v + 1];
+      u32 r = 0;
+      for (u32 j = b; j < e; j++) r += (w.st[w.adj[j]] == ST_R);
+      ed += e - b;
+      S[2 * i] = v;
+      S[2 * i + 1] = w.dR[v];
+      w.dR[v] = r;
+      if (c.on && i < w.nR && w.dC[v] + r <= c.kl) {
+        u32* wp = (u32*)((uintptr_t)(w.queued + v) & ~(uintptr_t)3);
+        u32 sh = ((uintptr_t)(w.queued + v) & 3) * 8;
+        u32 o2 = atomicOr(wp, 1u << sh);
+        if (!((o2 >> sh) & 0xffu)) c.next[atomicAdd(c.cnt, 1u)] = v;
+      }
+    }
+    w.edges += wsum(ed);
+    if (LANE == 0) S[2 * live_after] = live_after;
+    w.nsav += 2 * live_after + 1;
+    wsync();
+  }
+}
+
+__device__ void undo_batch(W& w, u32 start, u32 m, bool recount) {
+  u32* hist = w.L->hist;
+  const u32* E = w.trail + start;
+  if (recount) {
+    // restore the recounted dR values first (same live set as at the batch)
+    const u32 nl = w.sav[w.nsav - 1];
+    const u32* S = w.sav + w.nsav - 1 - 2 * nl;
+    for (u32 i = LANE; i < nl; i += 64) w.dR[S[2 * i]] = S[2 * i + 1];
+    w.nsav -= 2 * nl + 1;
+    wsync();
+  }
+  for (u32 i = LANE; i < m; i += 64) {
+    u32 v = E[i];
+    w.st[v] = ST_R;
+    atomicAdd(&hist[w.dC[v]], 1u);
+    w.rlist[w.nR + i] = v;
+    w.pos[v] = w.nR + i;
+  }
+  w.nR += m;
+  wsync();
+  if (recount) return;
+  if (m >= 8) {
+    u32 ed = 0;
+    for (u32 i = LANE; i < m; i += 64) {
+      u32 v = E[i];
+      const u32 b = w.off[v], e = w.off[v + 1];
+      ed += e - b;
+      for (u32 j = b; j < e; j++) {
+        u32 x = w.adj[j];
+        if (w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
+      }
+    }
+    w.edges += wsum(ed);
+  } else {
+    for (u32 i = 0; i < m; i++) {
+      u32 v = E[i];
+      const u32 b = w.off[v], e = w.off[v + 1];
+      for (u32 j = b + LANE; j < e; j += 64) {
+        u32 x = w.adj[j];
+        if (w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
+      }
+      w.edges += e - b;
+    }
+  }
   wsync();
 }
 
 __device__ void undo_to(W& w, u32 mark) {
   while (w.ntrail > mark) {
-    u32 e = w.trail[w.ntrail - 1];
-    u32 v = e & 0x7fffffffu;
-    bool wasC = (e & 0x80000000u) != 0;
-    // restore v first so that its own status is R while neighbours update
-    if (LANE == 0) {
-      w.st[v] = ST_R;
-      w.hist[w.dC[v]] += 1;
-      rlist_add(w, v);
-      if (wasC) w.nC--;
+    const u32 e = w.trail[w.ntrail - 1];
+    if (e & TR_MOVE) {
       w.ntrail--;
+      undo_move(w, e & TR_VMASK);
+    } else {
+      const u32 m = e & TR_MMASK;  // batch marker
+      const u32 start = w.ntrail - 1 - m;
+      w.ntrail = start;
+      undo_batch(w, start, m, (e & TR_RECOUNT) != 0);
     }
-    wsync();
-    const u32 b = w.off[v], en = w.off[v + 1];
-    for (u32 j = b + LANE; j < en; j += 64) {
-      u32 x = w.adj[j];
-      u8 s = w.st[x];
-      if (s == ST_X) continue;
-      if (wasC) {
-        u32 dc = w.dC[x];
-        if (s == ST_R) {
-          atomicSub(&w.hist[dc], 1u);
-          atomicAdd(&w.hist[dc - 1], 1u);
-        }
-        w.dC[x] = (u16)(dc - 1);
-      }
-      w.dR[x] += 1;
-    }
-    w.nR = bcast(w.nR);
-    w.nC = bcast(w.nC);
-    w.ntrail = bcast(w.ntrail);
-    wsync();
   }
 }
 
@@ -291,62 +468,72 @@ __device__ __forceinline__ u32 cur_kl(const W& w) { return aload(&w.Q->kl); }
 // ---------------------------------------------------------------------------
 // Node processing.
 // ---------------------------------------------------------------------------
-__device__ bool cond12(const W& w, u32 v, u32 Cs, u32 ubD, u32 kl) {
+__device__ __forceinline__ bool cond12(const W& w, u32 v, u32 Cs, u32 ubD, u32 kl) {
   u32 dc = w.dC[v], dr = w.dR[v];
   if ((w.flags & QF_RED1) && min(dr + dc, dc + w.h - Cs - 1) <= kl) return true;
   if ((w.flags & QF_RED2) && ubD < w.dist[v]) return true;
   return false;
 }
 
-// Worklist helpers: work[] holds queued vertices; count kept in *cnt (LDS).
-__device__ __forceinline__ void try_push(W& w, u32 x, u32 Cs, u32 ubD, u32 kl, u32* cnt) {
+__device__ __forceinline__ void try_push(W& w, u32 x, u32 Cs, u32 ubD, u32 kl, u32* list,
+                                         u32* cnt) {
   if (w.st[x] == ST_R && !w.queued[x] && cond12(w, x, Cs, ubD, kl)) {
-    // queued[] is a byte array: claim with a 32-bit CAS on the containing word
     u32* wp = (u32*)((uintptr_t)(w.queued + x) & ~(uintptr_t)3);
     u32 sh = ((uintptr_t)(w.queued + x) & 3) * 8;
     u32 old = atomicOr(wp, 1u << sh);
-    if (!((old >> sh) & 0xffu)) w.work[atomicAdd(cnt, 1u)] = x;
+    if (!((old >> sh) & 0xffu)) list[atomicAdd(cnt, 1u)] = x;
   }
 }
 
-// Rule 1/2 closure (see the CPU oracle's TrailSolver::r12 for the seeding
-// argument).  Returns true if anything was removed.
-__device__ bool r12(W& w, u32 from, bool full, u32 kl, u32* lds_cnt) {
+// Rule 1/2 closure.  At a parent node no R vertex met the condition after
+// its closure; in a child only these can: neighbours of vertices removed
+// since `from` (their dR dropped), vertices with dC <= kl + |C| + 1 - h
+// (|C| grew), or everything when kl changed.  Each round removes all
+// queued vertices as one batch and queues the neighbours it pushes below
+// the rule-1 threshold.  Returns true if anything was removed.
+__device__ bool r12(W& w, u32 from, bool full, u32 kl) {
+  Lds* L = w.L;
   const u32 Cs = w.nC;
   const u32 ubD = ubd_node(kl, w.h);
-  if (LANE == 0) *lds_cnt = 0;
+  if (LANE == 0) L->cnt = 0;
   wsync();
   if (!full && (w.flags & QF_RED1)) {
     long T = (long)kl + (long)Cs + 1 - (long)w.h;
     if (T >= 0) {
       u32 c = 0;
-      for (u32 d = LANE; d <= (u32)T && d < w.h + 2; d += 64) c += w.hist[d];
+      for (u32 d = LANE; d <= (u32)T && d < w.h + 3; d += 64) c += L->hist[d];
       if (wsum(c)) full = true;
     }
   }
   if (full) {
-    for (u32 i = LANE; i < w.nR; i += 64) try_push(w, w.rlist[i], Cs, ubD, kl, lds_cnt);
+    for (u32 i = LANE; i < w.nR; i += 64) try_push(w, w.rlist[i], Cs, ubD, kl, w.work, &L->cnt);
   } else {
     for (u32 t = from; t < w.ntrail; t++) {
-      u32 e = w.trail[t];
-      if (e & 0x80000000u) continue;
+      const u32 e = w.trail[t];
+      if (e & (TR_MOVE | TR_MARK)) continue;
       const u32 b = w.off[e], en = w.off[e + 1];
-      for (u32 j = b + LANE; j < en; j += 64) try_push(w, w.adj[j], Cs, ubD, kl, lds_cnt);
+      for (u32 j = b + LANE; j < en; j += 64) try_push(w, w.adj[j], Cs, ubD, kl, w.work, &L->cnt);
+      w.edges += en - b;
     }
   }
   wsync();
-  u32 head = 0;
-  u32 cnt = bcast(*lds_cnt);
-  bool removed = cnt > 0;
-  while (head < cnt) {
-    u32 v = w.work[head];
-    head++;
-    if (LANE == 0) w.queued[v] = 0;
-    remove_r(w, v);
-    const u32 b = w.off[v], en = w.off[v + 1];
-    for (u32 j = b + LANE; j < en; j += 64) try_push(w, w.adj[j], Cs, ubD, kl, lds_cnt);
+  u32 cnt = bcast(L->cnt);
+  const bool removed = cnt > 0;
+  u32* cur = w.work;
+  u32* nxt = w.work2;
+  while (cnt) {
+    if (LANE == 0) L->cnt2 = 0;
     wsync();
-    cnt = bcast(*lds_cnt);
+    Clo c;
+    c.on = (w.flags & QF_RED1) ? 1u : 0u;
+    c.kl = kl;
+    c.next = nxt;
+    c.cnt = &L->cnt2;
+    remove_batch(w, cur, cnt, c);
+    cnt = bcast(L->cnt2);
+    u32* t = cur;
+    cur = nxt;
+    nxt = t;
   }
   return removed;
 }
@@ -364,7 +551,9 @@ __device__ bool tight_exists(const W& w, u32 kl) {
 // smallest entries for each of the k largest R->C degrees (src/helpers.cc:
 // 1066-1101 with a correct sort).  Values are capped at h+1: a minimum of
 // h+1 or more can never be <= kl, so the prune decision is unchanged.
-__device__ u32 unr_bound(W& w, u32 k, u32* c) {
+__device__ u32 unr_bound(W& w, u32 k) {
+  u32* c = w.L->chist;
+  const u32* hist = w.L->hist;
   const u32 h = w.h;
   const u32 top = h + 1;
   for (u32 i = LANE; i <= top; i += 64) c[i] = 0;
@@ -377,9 +566,9 @@ __device__ u32 unr_bound(W& w, u32 k, u32* c) {
     while (lo < top && c[lo] == 0) lo++;
     const u32 m = w.nC;
     int d = (int)top;
-    u32 left = w.hist[d];
+    u32 left = hist[d];
     for (u32 i = 0; i < k; i++) {
-      while (d > 0 && left == 0) left = w.hist[--d];
+      while (d > 0 && left == 0) left = hist[--d];
       if (d <= 0) break;  // the remaining R vertices have dC == 0
       left--;
       const u32 x = min((u32)d, m);
@@ -412,17 +601,8 @@ struct NodeOut {
   u32 klc;  // kl at the end of the closure
 };
 
-// lds scratch layout per workgroup
-struct Lds {
-  u32 cnt;
-  u32 nfront;
-  u32 ncand;
-  u32 pad;
-  u32 D[SCS_FRAME_DOMS];
-  u32 chist[SCS_HCAP + 2];
-};
-
-__device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
+__device__ NodeOut process(W& w, u32 from, u32 kl_seen) {
+  Lds* L = w.L;
   NodeOut o;
   o.branch = 0;
   o.nD = 0;
@@ -436,9 +616,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
       u32 mn = INV;
       for (u32 i = LANE; i < Cs; i += 64) mn = min(mn, (u32)w.dC[w.clist[i]]);
       mn = wmin(mn);
-      if (mn != INV && mn >= kl) {
-        offer(w, mn);
-      }
+      if (mn != INV && mn >= kl) offer(w, mn);
       u32 nk = cur_kl(w);
       if (nk != kl) full = true;
       kl = nk;
@@ -448,7 +626,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
       return o;
     }
     if ((w.flags & QF_RED3) && tight_exists(w, kl)) {
-      // collect tight C vertices first (their dC+dR is invariant under moves)
+      // tight C vertices first (their dC+dR is invariant under moves)
       if (LANE == 0) L->cnt = 0;
       wsync();
       for (u32 i = LANE; i < Cs; i += 64) {
@@ -456,7 +634,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
         if (w.dC[u] + w.dR[u] == kl + 1 && w.dR[u] > 0) w.work[atomicAdd(&L->cnt, 1u)] = u;
       }
       wsync();
-      u32 nt = bcast(L->cnt);
+      const u32 nt = bcast(L->cnt);
       bool over = false;
       for (u32 t = 0; t < nt && !over; t++) {
         u32 u = w.work[t];
@@ -478,7 +656,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
       }
       continue;
     }
-    bool removed = r12(w, from, full, kl, &L->cnt);
+    const bool removed = r12(w, from, full, kl);
     from = w.ntrail;
     full = false;
     if (removed && (w.flags & QF_RED3) && tight_exists(w, kl)) continue;
@@ -498,10 +676,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
     }
     ub = wmin(b);
   }
-  if ((w.flags & QF_PRUN2) && ub > kl) {
-    u32 k = min(h - Cs, w.nR);
-    ub = min(ub, unr_bound(w, k, L->chist));
-  }
+  if ((w.flags & QF_PRUN2) && ub > kl) ub = min(ub, unr_bound(w, min(h - Cs, w.nR)));
   if (ub <= kl) return o;
   // connection scores by scattering from C into acc[]; frontier = R with dC>0
   if (LANE == 0) L->nfront = 0;
@@ -517,6 +692,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
       u64 old = atomicAdd((unsigned long long*)&w.acc[x], (unsigned long long)r);
       if (old == 0) w.front[atomicAdd(&L->nfront, 1u)] = x;
     }
+    w.edges += e - b;
   }
   wsync();
   const u32 nf = bcast(L->nfront);
@@ -531,9 +707,8 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
       bid = x;
     }
   }
-  u64 gbest = wmax64(best);
-  u32 cand = (best == gbest) ? bid : INV;
-  const u32 us = wmin(cand);
+  const u64 gbest = wmax64(best);
+  const u32 us = wmin(best == gbest ? bid : INV);
   if (nf == 0 || us == INV) {
     for (u32 i = LANE; i < nf; i += 64) w.acc[w.front[i]] = 0;
     wsync();
@@ -545,10 +720,9 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
     const u32 lu = w.dC[us] + w.dR[us];
     const u32 ub0 = w.off[us], ue = w.off[us + 1];
     for (u32 j = ub0 + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 1;
-    wsync();
     if (LANE == 0) L->ncand = 0;
     wsync();
-    // each lane tests one frontier vertex
+    u32 ed = 0;
     for (u32 i = LANE; i < nf; i += 64) {
       u32 x = w.front[i];
       if (x == us || w.dC[x] + w.dR[x] > lu + 1) continue;
@@ -556,6 +730,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
       const u32 b = w.off[x], e = w.off[x + 1];
       for (u32 j = b; j < e; j++) {
         u32 y = w.adj[j];
+        ed++;
         if (w.st[y] == ST_X) continue;
         if (y != us && !w.nmark[y]) {
           dom = false;
@@ -564,11 +739,12 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
       }
       if (dom) w.work[atomicAdd(&L->ncand, 1u)] = x;
     }
+    w.edges += wsum(ed) + (ue - ub0);
     wsync();
     for (u32 j = ub0 + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 0;
     const u32 nc = bcast(L->ncand);
-    // select the top limit_doms by (score desc, id asc); score uses the
-    // C neighbours with dC != 0 (only C = {q} with dC(q) = 0 differs)
+    // top limit_doms by (score desc, id asc); the score counts C neighbours
+    // with dC != 0 (only C = {q} with dC(q) = 0 differs from acc)
     const u32 want = min(nc, w.limit_doms);
     for (u32 t = 0; t < want; t++) {
       u64 bs = 0;
@@ -582,14 +758,9 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, Lds* L) {
           bx = x;
         }
       }
-      // pack: compare score first, then smaller id
-      u64 gb = 0;
-      {
-        u64 key = (bx == INV) ? 0ull : bs + 1;  // +1 keeps valid keys above 0
-        gb = wmax64(key);
-      }
-      u32 c2 = (bx != INV && bs + 1 == gb) ? bx : INV;
-      u32 pick = wmin(c2);
+      const u64 key = (bx == INV) ? 0ull : bs + 1;
+      const u64 gb = wmax64(key);
+      const u32 pick = wmin((bx != INV && key == gb) ? bx : INV);
       if (LANE == 0) L->D[t] = pick;
       for (u32 i = LANE; i < nc; i += 64)
         if (w.work[i] == pick) w.work[i] = INV;
@@ -652,18 +823,15 @@ __device__ int pool_try_dequeue(const Pool& P, u64* out_pos) {
 }
 
 // Rebuild the wave's state from a task payload.
-__device__ void rebuild(W& w, const u32* task, Lds* L) {
+__device__ void rebuild(W& w, const u32* task) {
   const u32 nC = task[1], nL = task[2], form = task[3];
   const u32* C = task + 4;
   const u32* Lst = task + 4 + nC;
   const u32 K = w.K;
   const u8 fill = form ? ST_X : ST_R;
-  for (u32 v = LANE; v < K; v += 64) {
-    w.st[v] = fill;
-    w.queued[v] = 0;
-    w.nmark[v] = 0;
-    w.acc[v] = 0;
-  }
+  // queued[], nmark[] and acc[] are all-zero between nodes (every node
+  // clears what it set), so only the status needs initialising
+  for (u32 v = LANE; v < K; v += 64) w.st[v] = fill;
   wsync();
   for (u32 i = LANE; i < nL; i += 64) w.st[Lst[i]] = form ? ST_R : ST_X;
   wsync();
@@ -671,7 +839,7 @@ __device__ void rebuild(W& w, const u32* task, Lds* L) {
     w.st[C[i]] = ST_C;
     w.clist[i] = C[i];
   }
-  for (u32 i = LANE; i < w.h + 2; i += 64) w.hist[i] = 0;
+  for (u32 i = LANE; i < SCS_HCAP + 4; i += 64) w.L->hist[i] = 0;
   wsync();
   // R list by ballot compaction (ascending ids)
   u32 base = 0;
@@ -690,47 +858,44 @@ __device__ void rebuild(W& w, const u32* task, Lds* L) {
   w.nC = nC;
   wsync();
   // degrees of live vertices, one lane per vertex
+  u32 ed = 0;
   for (u32 i = LANE; i < w.nR + nC; i += 64) {
     u32 v = i < w.nR ? w.rlist[i] : w.clist[i - w.nR];
     u32 c = 0, r = 0;
-    for (u32 j = w.off[v]; j < w.off[v + 1]; j++) {
+    const u32 b = w.off[v], e = w.off[v + 1];
+    for (u32 j = b; j < e; j++) {
       u8 s = w.st[w.adj[j]];
       c += (s == ST_C);
       r += (s == ST_R);
     }
+    ed += e - b;
     w.dC[v] = (u16)c;
     w.dR[v] = r;
-    if (i < w.nR) atomicAdd(&w.hist[min(c, w.h + 1)], 1u);
+    if (i < w.nR) atomicAdd(&w.L->hist[min(c, w.h + 1)], 1u);
   }
+  w.edges += wsum(ed);
   w.ntrail = 0;
   w.nframes = 0;
+  w.nsav = 0;
   wsync();
 }
 
-// Serialise the Z child of frame fi (the unexplored "exclude u*" branch)
-// into dst; returns the payload length in words.
-__device__ u32 serialize_z(W& w, u32 fi, u32* dst, u32 qslot, u32 cap, Lds* L) {
+// Serialises the Z child of frame fi (the unexplored "exclude u*" branch)
+// into dst (dst[3] = form set by the caller: 1 = list R, 0 = list X);
+// returns the payload length in words.
+__device__ u32 serialize_z(W& w, u32 fi, u32* dst, u32 qslot) {
   const Frame& f = w.frames[fi];
   // mark vertices changed after f.mark (they were R at the frame)
-  for (u32 t = f.mark + LANE; t < w.ntrail; t += 64) w.nmark[w.trail[t] & 0x7fffffffu] = 1;
+  for (u32 t = f.mark + LANE; t < w.ntrail; t += 64) {
+    const u32 e = w.trail[t];
+    if (!(e & TR_MARK)) w.nmark[e & TR_VMASK] = 1;
+  }
   wsync();
   for (u32 i = LANE; i < f.nD; i += 64) w.nmark[f.D[i]] = 2;
   if (LANE == 0) w.nmark[f.ustar] = 2;
   wsync();
-  // R at Z = (current R) + (changed after mark) - ustar - D
-  const u32 nRz = w.nR + (w.ntrail - f.mark) - 1 - f.nD;
-  const u32 nXz = w.K - f.csize - nRz;
-  const u32 form = nRz <= nXz ? 1u : 0u;
-  const u32 nL = form ? nRz : nXz;
-  if (4 + f.csize + nL > cap) {
-    for (u32 t = f.mark + LANE; t < w.ntrail; t += 64) w.nmark[w.trail[t] & 0x7fffffffu] = 0;
-    wsync();
-    for (u32 i = LANE; i < f.nD; i += 64) w.nmark[f.D[i]] = 0;
-    if (LANE == 0) w.nmark[f.ustar] = 0;
-    wsync();
-    return 0;
-  }
   for (u32 i = LANE; i < f.csize; i += 64) dst[4 + i] = w.clist[i];
+  const u32 form = dst[3];
   u32* out = dst + 4 + f.csize;
   u32 base = 0;
   for (u32 v0 = 0; v0 < w.K; v0 += 64) {
@@ -742,8 +907,7 @@ __device__ u32 serialize_z(W& w, u32 fi, u32* dst, u32 qslot, u32 cap, Lds* L) {
       u8 s = w.st[v], m = w.nmark[v];
       bool inRz = (m == 1) || (m == 0 && s == ST_R);
       bool inC = (m == 0 && s == ST_C);
-      bool inXz = !inRz && !inC;
-      take = form ? inRz : inXz;
+      take = form ? inRz : (!inRz && !inC);
     }
     u64 bm = __ballot(take);
     if (take) out[base + __popcll(bm & lanes_below())] = v;
@@ -753,10 +917,12 @@ __device__ u32 serialize_z(W& w, u32 fi, u32* dst, u32 qslot, u32 cap, Lds* L) {
     dst[0] = qslot;
     dst[1] = f.csize;
     dst[2] = base;
-    dst[3] = form;
   }
   wsync();
-  for (u32 t = f.mark + LANE; t < w.ntrail; t += 64) w.nmark[w.trail[t] & 0x7fffffffu] = 0;
+  for (u32 t = f.mark + LANE; t < w.ntrail; t += 64) {
+    const u32 e = w.trail[t];
+    if (!(e & TR_MARK)) w.nmark[e & TR_VMASK] = 0;
+  }
   wsync();
   for (u32 i = LANE; i < f.nD; i += 64) w.nmark[f.D[i]] = 0;
   if (LANE == 0) w.nmark[f.ustar] = 0;
@@ -764,7 +930,9 @@ __device__ u32 serialize_z(W& w, u32 fi, u32* dst, u32 qslot, u32 cap, Lds* L) {
   return 4 + f.csize + base;
 }
 
-__device__ void save_hdr(W& w, u32 active, u32 qslot, u32 from, u32 kl_seen, u32 pend) {
+__device__ void save_state(W& w, u32 active, u32 qslot, u32 from, u32 kl_seen) {
+  if (active)
+    for (u32 i = LANE; i < SCS_HCAP + 4; i += 64) w.ghist[i] = w.L->hist[i];
   if (LANE == 0) {
     WaveHdr* H = w.hdr;
     H->active = active;
@@ -775,7 +943,8 @@ __device__ void save_hdr(W& w, u32 active, u32 qslot, u32 from, u32 kl_seen, u32
     H->nR = w.nR;
     H->from = from;
     H->kl_seen = kl_seen;
-    H->pend_nodes = pend;
+    H->pend_nodes = 0;
+    H->nsav = w.nsav;
   }
   wsync();
 }
@@ -793,6 +962,18 @@ __device__ void finish_task(W& w, const EngineArgs& A, u32 qslot) {
   wsync();
 }
 
+__device__ void flush_counters(W& w, const EngineArgs& A, u32& pend) {
+  if (LANE == 0) {
+    if (pend) {
+      atomicAdd((unsigned long long*)&w.Q->nodes, (unsigned long long)pend);
+      atomicAdd((unsigned long long*)&A.glob->nodes, (unsigned long long)pend);
+    }
+    if (w.edges) atomicAdd((unsigned long long*)&A.glob->edges, (unsigned long long)w.edges);
+  }
+  pend = 0;
+  w.edges = 0;
+}
+
 // ---------------------------------------------------------------------------
 // The persistent epoch kernel.
 // ---------------------------------------------------------------------------
@@ -803,16 +984,19 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
   const u64 t0 = wall_clock64();
   const u64 deadline = t0 + A.slice_ticks;
   W w;
-  bind(w, A, wave);
+  bind(w, A, wave, &L);
   WaveHdr* H = w.hdr;
   u32 active = H->active, qslot = H->qslot;
-  u32 from = H->from, kl_seen = H->kl_seen, pend = H->pend_nodes;
+  u32 from = H->from, kl_seen = H->kl_seen, pend = 0;
   if (active) {
     bind_query(w, &A.slots[qslot]);
     w.nframes = H->nframes;
     w.ntrail = H->trail;
     w.nC = H->nC;
     w.nR = H->nR;
+    w.nsav = H->nsav;
+    for (u32 i = LANE; i < SCS_HCAP + 4; i += 64) L.hist[i] = w.ghist[i];
+    wsync();
   }
   bool idle_counted = false;
   u32 idle_spins = 0;
@@ -850,7 +1034,7 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
         finish_task(w, A, qslot);
         continue;
       }
-      rebuild(w, task, &L);
+      rebuild(w, task);
       __threadfence();
       if (LANE == 0) {
         astore64(&A.pool.seq[pos % A.pool.nslots], pos + A.pool.nslots);
@@ -880,44 +1064,32 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
     quit = bcast(quit);
     if (quit) {
       active = 0;
-      if (pend) {
-        if (LANE == 0) {
-          atomicAdd((unsigned long long*)&Q->nodes, (unsigned long long)pend);
-          atomicAdd((unsigned long long*)&A.glob->nodes, (unsigned long long)pend);
-        }
-        pend = 0;
-      }
+      flush_counters(w, A, pend);
       finish_task(w, A, qslot);
       continue;
     }
-    NodeOut o = process(w, from, kl_seen, &L);
+    NodeOut o = process(w, from, kl_seen);
     pend++;
-    if (pend >= A.node_flush) {
-      if (LANE == 0) {
-        atomicAdd((unsigned long long*)&Q->nodes, (unsigned long long)pend);
-        atomicAdd((unsigned long long*)&A.glob->nodes, (unsigned long long)pend);
-      }
-      pend = 0;
-      wsync();
-    }
+    if (pend >= A.node_flush) flush_counters(w, A, pend);
     if (o.branch) {
-      u32 fi = w.nframes;
+      const u32 fi = w.nframes;
+      Frame* f = &w.frames[fi];
       if (LANE == 0) {
-        Frame& f = w.frames[fi];
-        f.mark = w.ntrail;
-        f.ustar = o.ustar;
-        f.nD = o.nD;
-        f.next = 0;
-        f.kl = o.klc;
-        f.csize = w.nC;
-        f.zgone = 0;
+        f->mark = w.ntrail;
+        f->ustar = o.ustar;
+        f->nD = o.nD;
+        f->next = 0;
+        f->kl = o.klc;
+        f->csize = w.nC;
+        f->zgone = 0;
       }
-      for (u32 i = LANE; i < o.nD; i += 64) w.frames[fi].D[i] = L.D[i];
+      for (u32 i = LANE; i < o.nD; i += 64) f->D[i] = L.D[i];
       wsync();
       w.nframes = fi + 1;
       from = w.ntrail;
       kl_seen = o.klc;
-      for (u32 i = 0; i < o.nD; i++) remove_r(w, L.D[i]);
+      const Clo none{0, 0, nullptr, nullptr};
+      remove_batch(w, f->D, o.nD, none);
       move_c(w, o.ustar);
     } else {
       // backtrack to the next pending child
@@ -927,12 +1099,14 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
         undo_to(w, f->mark);
         from = f->mark;
         kl_seen = f->kl;
-        u32 next = f->next + 1;
+        const u32 next = f->next + 1;
         if (LANE == 0) f->next = next;
         wsync();
-        if (next <= f->nD) {
-          u32 i = next - 1;
-          for (u32 j = 0; j < i; j++) remove_r(w, f->D[j]);
+        const u32 nD = f->nD;
+        const Clo none{0, 0, nullptr, nullptr};
+        if (next <= nD) {
+          const u32 i = next - 1;
+          remove_batch(w, f->D, i, none);
           move_c(w, f->ustar);
           move_c(w, f->D[i]);
           cont = true;
@@ -942,23 +1116,18 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
           w.nframes--;
           continue;
         }
-        Frame z = *f;
+        // Z: drop ustar and D
+        if (LANE == 0) w.work[0] = f->ustar;
+        for (u32 i = LANE; i < nD; i += 64) w.work[1 + i] = f->D[i];
+        wsync();
         w.nframes--;
-        remove_r(w, z.ustar);
-        for (u32 j = 0; j < z.nD; j++) remove_r(w, z.D[j]);
+        remove_batch(w, w.work, nD + 1, none);
         cont = true;
         break;
       }
       if (!cont) {
         active = 0;
-        if (pend) {
-          if (LANE == 0) {
-            atomicAdd((unsigned long long*)&Q->nodes, (unsigned long long)pend);
-            atomicAdd((unsigned long long*)&A.glob->nodes, (unsigned long long)pend);
-          }
-          pend = 0;
-          wsync();
-        }
+        flush_counters(w, A, pend);
         finish_task(w, A, qslot);
         continue;
       }
@@ -976,9 +1145,15 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
         }
         if (fi != INV) {
           const Frame& f = w.frames[fi];
-          const u32 nRz = w.nR + (w.ntrail - f.mark) - 1 - f.nD;
+          // R at Z = current R + every vertex changed after the frame's
+          // mark (markers excluded) - ustar - D
+          u32 chg = 0;
+          for (u32 t = f.mark + LANE; t < w.ntrail; t += 64) chg += !(w.trail[t] & TR_MARK);
+          chg = wsum(chg);
+          const u32 nRz = w.nR + chg - 1 - f.nD;
           const u32 nXz = w.K - f.csize - nRz;
-          const u32 need = 4 + f.csize + min(nRz, nXz);
+          const u32 form = nRz <= nXz ? 1u : 0u;
+          const u32 need = 4 + f.csize + (form ? nRz : nXz);
           u64 pos = 0;
           u32 got = 0;
           if (need <= A.pool.slot_words && LANE == 0) got = pool_try_enqueue(A.pool, &pos);
@@ -987,7 +1162,9 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
             u32 plo = __shfl((u32)pos, 0), phi = __shfl((u32)(pos >> 32), 0);
             pos = ((u64)phi << 32) | plo;
             u32* dst = A.pool.data + (pos % A.pool.nslots) * (u64)A.pool.slot_words;
-            serialize_z(w, fi, dst, qslot, A.pool.slot_words, &L);
+            if (LANE == 0) dst[3] = form;
+            wsync();
+            serialize_z(w, fi, dst, qslot);
             if (LANE == 0) {
               w.frames[fi].zgone = 1;
               atomicAdd(&Q->pending, 1u);
@@ -1005,9 +1182,10 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
   }
   if (idle_counted && LANE == 0) atomicSub(&A.glob->idle, 1u);
   if (active) {
-    save_hdr(w, 1, qslot, from, kl_seen, pend);
+    flush_counters(w, A, pend);
+    save_state(w, 1, qslot, from, kl_seen);
   } else {
-    save_hdr(w, 0, 0, 0, INV, 0);
+    save_state(w, 0, 0, 0, INV);
   }
 }
 

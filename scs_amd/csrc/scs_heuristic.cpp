@@ -10,7 +10,9 @@
 //     vertex's neighbours in H; each vertex keeps its H-neighbours in
 //     ascending order, so the sum has the same terms in the same order as
 //     the reference's adjacency scan (bit-identical doubles);
-//   heu4 (inc/heuristic.h:242-317): unchanged.
+//   heu4 (inc/heuristic.h:242-317): unchanged except that q's +1
+//     (inc/heuristic.h:259) is not counted in the min-degree test when q is
+//     popped alone (l <= 1); the reference then reports 1 for an isolated q.
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -199,6 +201,9 @@ static void heu4(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
     Q.pop();
     if (!W.inS[v]) continue;
     uint32_t remain = (uint32_t)S.size() - (uint32_t)rmv.size();
+    // q's +1 (inc/heuristic.h:259) only keeps it from being peeled early;
+    // it is popped only when alone, where its true degree is 0
+    if (v == q) d -= 1;
     if (remain >= N1 && remain <= N2 && d > mindeg) {
       mindeg = d;
       rmv_idx = (uint32_t)rmv.size();

@@ -82,12 +82,15 @@ struct WaveHdr {
   u32 kl_seen;   // kl at the parent's closure (0xffffffff = rescan all)
   u32 K;         // kept-set size of the bound query
   u32 pend_nodes;// nodes not yet added to the query counter
-  u32 pad[6];
+  u32 pend_edges;// adjacency entries not yet added to the global counter
+  u32 nsav;      // words on the recount save stack
+  u32 pad[4];
 };
 
 // Offsets (bytes) of each per-wave array inside one wave's region.
 struct WaveLayout {
-  u64 st, nmark, queued, dC, dR, pos, rlist, acc, work, front, trail, frames, clist, hist, hdr;
+  u64 st, nmark, queued, dC, dR, pos, rlist, acc, work, work2, front, aux, trail, sav, frames,
+      clist, hist, hdr;
   u64 stride;
 };
 
@@ -108,8 +111,9 @@ struct EngineGlobals {
   u64 donations;
   u64 rebuilds;
   u64 nodes;
+  u64 edges;       // adjacency entries scanned by search waves
   u32 lock_fail;
-  u32 pad[7];
+  u32 pad[5];
 };
 
 struct EngineArgs {

@@ -60,7 +60,7 @@ class Stats(C.Structure):
     _fields_ = [("nodes", C.c_uint64), ("donations", C.c_uint64), ("rebuilds", C.c_uint64),
                 ("engine_us", C.c_double), ("prep_us", C.c_double), ("heu_us", C.c_double),
                 ("prep_bytes", C.c_uint64), ("epochs", C.c_uint64), ("waves", C.c_uint32),
-                ("kcap", C.c_uint32)]
+                ("kcap", C.c_uint32), ("edges", C.c_uint64), ("core_us", C.c_double)]
 
 
 def lib():

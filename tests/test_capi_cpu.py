@@ -1,0 +1,73 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+declared in include/scs.h, and its host heuristic reproduces the oracle's
+restatement of CSSC_heu (kl and H) exactly."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleGraph
+from scs_amd import configs, graphgen as gg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "scs_amd", "libscs.so")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "scs.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[a-z_ ]+[ \*]+(scs_[a-z_]+)\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail("libscs.so not built: run python scs_amd/build.py")
+    return C.CDLL(LIB)
+
+
+def test_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def host_heuristic(lib, off, adj, l, h, q):
+    kl = C.c_uint32()
+    hs = C.c_uint32()
+    H = np.zeros(len(off) + h + 1, np.uint32)
+    f = lib.scs_heuristic_csr
+    f.argtypes = [np.ctypeslib.ndpointer(np.int64, flags="C"), np.ctypeslib.ndpointer(np.int32, flags="C"),
+                  C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
+                  np.ctypeslib.ndpointer(np.uint32, flags="C"), C.c_uint32, C.POINTER(C.c_uint32)]
+    rc = f(off, adj, len(off) - 1, l, h, q, C.byref(kl), H, H.size, C.byref(hs))
+    assert rc == 0
+    return kl.value, H[:hs.value]
+
+
+@pytest.mark.parametrize("gname", ["100.txt", "hepPH_SCS", "C1"])
+def test_host_heuristic_equals_oracle(lib, gname):
+    if gname == "C1":
+        g = configs.graph("C1")
+        og = OracleGraph.from_csr(g.n, g.off, g.adj)
+        core = og.core()
+        qs = [(x.l, x.h, x.q) for x in configs.queries("C1", core.astype(np.int64))[:40]]
+    else:
+        og = OracleGraph.load(os.path.join(GOLD, gname))
+        rng = np.random.default_rng(3)
+        qs = []
+        for _ in range(40):
+            l = int(rng.integers(1, 20))
+            qs.append((l, l + int(rng.integers(0, 20)), int(rng.integers(0, og.n))))
+    off, adj = og.csr()
+    for l, h, q in qs:
+        a = host_heuristic(lib, off, adj, l, h, q)
+        b = og.heuristic(0, l, h, q)
+        assert a[0] == b[0], (gname, l, h, q)
+        assert list(a[1]) == list(b[1]), (gname, l, h, q)
