@@ -241,15 +241,22 @@ class Engine:
         except Exception:
             pass
 
-    def run(self, queries, tags=None):
-        """Submits all queries, drains the engine and returns results by tag."""
+    def run(self, queries, tags=None, progress=None):
+        """Submits all queries, drains the engine and returns results by tag.
+        progress: seconds between progress lines on stderr (None = quiet)."""
+        import sys
+        import time
         for i, qq in enumerate(queries):
             l, h, q, heu, ld = qq
             self.submit(l, h, q, heu, ld, i if tags is None else tags[i])
         res = []
+        last = time.time()
         while True:
             left = self.step()
             res.extend(self.poll())
+            if progress is not None and time.time() - last > progress:
+                last = time.time()
+                print(f"[scs] {len(res)}/{len(queries)} queries done", file=sys.stderr, flush=True)
             if left == 0:
                 break
         res.extend(self.poll())
