@@ -89,7 +89,7 @@ typedef struct scs_params {
   uint32_t deterministic;  /* 1 = a single wave and no work sharing: node
                               counts equal the CPU oracle's */
   uint32_t learned;        /* 1 = MLP scores the branching vertex */
-  const void* model;       /* MLP weights (scs_model layout) when learned */
+  const void* model;       /* const scs_model* used when learned */
   uint32_t heuristic_threads; /* host threads for CSSC_heu (0 = 4) */
   uint32_t reserved[7];
 } scs_params;
@@ -176,6 +176,46 @@ int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_t n, uint32
 int scs_prep_probe(scs_engine* e, uint32_t q, uint32_t kl0, uint32_t ubD, uint32_t* K,
                    uint64_t* nnz, uint32_t* kept, uint32_t* loff, uint32_t* dist,
                    uint32_t kept_cap, uint32_t* ladj, uint64_t adj_cap);
+
+/* Branching MLP (6 features -> 64 -> 64 -> 1, ReLU).  No counterpart in
+ * the reference: it replaces the connection score of selectUstar
+ * (src/helpers.cc:1104-1168) as the order of exploration only.  Features
+ * of a candidate v: dC(v), dR(v), dist(q, v), core(v), slack =
+ * dC + min(dR, h-|C|-1) - kl, and its plain connection score (sum of
+ * 1/dC(u) over C neighbours u plus dR/dmax); inputs are normalised as
+ * (x - mean) * scale.  FP32 master weights; the engine runs them in BF16 on
+ * the MFMA units with FP32 accumulation. */
+typedef struct scs_model {
+  float w1[8][64]; /* rows 6 and 7 unused (zero) */
+  float b1[64];
+  float w2[64][64];
+  float b2[64];
+  float w3[64];
+  float b3;
+  float mean[8];
+  float scale[8];
+  float pad[7];
+} scs_model;
+
+int scs_model_init(scs_model* m, uint64_t seed);
+/* Mean weighted binary cross-entropy gradient over n rows (X: n x 6 raw
+ * features, y: 0/1 labels), computed by the GPU training kernels. */
+int scs_mlp_grad(const scs_model* m, const float* X, const float* y, uint32_t n,
+                 float pos_weight, scs_model* grad, float* loss);
+/* Mini-batch Adam training on the GPU; sets mean/scale from the data. */
+int scs_mlp_train(scs_model* m, const float* X, const float* y, uint32_t n, uint32_t epochs,
+                  uint32_t batch, float lr, float pos_weight, uint64_t seed, float* final_loss);
+/* Scores n rows with the engine's BF16 MFMA inference routine. */
+int scs_mlp_infer(const scs_model* m, const float* X, uint32_t n, float* scores);
+/* Installs (or with NULL removes) the model used by queries of this engine
+ * when scs_params.learned is set. */
+int scs_engine_set_model(scs_engine* e, const scs_model* m, uint32_t learned);
+/* Search traces: every stride-th node of a query records up to 64 of its
+ * branching candidates as rows {tag, vertex (original id), dC, dR, dist,
+ * core, slack, connection score (float bits)}.  capacity_rows = 0 disables tracing. */
+int scs_engine_trace(scs_engine* e, uint32_t capacity_rows, uint32_t stride);
+/* Copies up to max_rows rows (8 uint32 each) and clears the trace. */
+int scs_engine_trace_read(scs_engine* e, uint32_t* rows, uint32_t max_rows, uint32_t* n);
 
 const char* scs_strerror(int code);
 

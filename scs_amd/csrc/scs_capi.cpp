@@ -18,6 +18,9 @@
 
 #include "scs_host.h"
 #include "scs_internal.h"
+#include "scs_mlp.h"
+
+void scs_pack_model(const scs_model* m, MlpDev* d);
 
 extern "C" {
 hipError_t scs_launch_core_init(const i64* off, u32 n, u32* deg, u32* core, hipStream_t s);
@@ -339,6 +342,11 @@ struct scs_engine {
   u32* d_admit = nullptr;
   unsigned long long* d_prepbytes = nullptr;
   u8* d_arena = nullptr;
+  MlpDev* d_model = nullptr;
+  bool have_model = false;
+  u32* d_trace = nullptr;
+  u32* d_trace_cnt = nullptr;
+  u32 trace_cap = 0;
   Arena arena;
   u32 stamp_id = 0;
   u32* h_cnt = nullptr;  // pinned
@@ -499,7 +507,63 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   E->heu_work.resize(p.heuristic_threads);
   E->st.waves = E->nwaves;
   E->st.kcap = E->kcap;
+  if (hipMalloc(&E->d_model, sizeof(MlpDev)) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMemset(E->d_model, 0, sizeof(MlpDev)) != hipSuccess) return fail(SCS_EHIP);
+  if (hipMalloc(&E->d_trace_cnt, sizeof(u32)) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMemset(E->d_trace_cnt, 0, sizeof(u32)) != hipSuccess) return fail(SCS_EHIP);
+  A.model = nullptr;
+  A.trace = nullptr;
+  A.trace_cnt = E->d_trace_cnt;
+  A.trace_cap = 0;
+  A.trace_stride = 1;
+  if (p.learned && p.model) {
+    int rc = scs_engine_set_model(E, (const scs_model*)p.model, 1);
+    if (rc) return fail(rc);
+  }
   *out = E;
+  return SCS_OK;
+}
+
+extern "C" int scs_engine_set_model(scs_engine* E, const scs_model* m, uint32_t learned) {
+  if (!E) return SCS_EINVAL;
+  if (m) {
+    MlpDev h;
+    scs_pack_model(m, &h);
+    HIPCHK(hipMemcpy(E->d_model, &h, sizeof(MlpDev), hipMemcpyHostToDevice));
+    E->have_model = true;
+  }
+  E->p.learned = learned && E->have_model ? 1u : 0u;
+  E->args.model = E->p.learned ? E->d_model : nullptr;
+  return SCS_OK;
+}
+
+extern "C" int scs_engine_trace(scs_engine* E, uint32_t capacity_rows, uint32_t stride) {
+  if (!E) return SCS_EINVAL;
+  if (E->d_trace) {
+    HIPCHK(hipFree(E->d_trace));
+    E->d_trace = nullptr;
+  }
+  E->trace_cap = 0;
+  if (capacity_rows) {
+    HIPCHK(hipMalloc(&E->d_trace, sizeof(u32) * 8ull * capacity_rows));
+    E->trace_cap = capacity_rows;
+  }
+  HIPCHK(hipMemset(E->d_trace_cnt, 0, sizeof(u32)));
+  E->args.trace = E->d_trace;
+  E->args.trace_cap = E->trace_cap;
+  E->args.trace_stride = stride ? stride : 1;
+  return SCS_OK;
+}
+
+extern "C" int scs_engine_trace_read(scs_engine* E, uint32_t* rows, uint32_t max_rows, uint32_t* n) {
+  if (!E || !n) return SCS_EINVAL;
+  u32 cnt = 0;
+  HIPCHK(hipMemcpy(&cnt, E->d_trace_cnt, sizeof(u32), hipMemcpyDeviceToHost));
+  cnt = std::min(cnt, E->trace_cap);
+  const u32 k = std::min(cnt, max_rows);
+  if (k && rows) HIPCHK(hipMemcpy(rows, E->d_trace, sizeof(u32) * 8ull * k, hipMemcpyDeviceToHost));
+  *n = k;
+  HIPCHK(hipMemset(E->d_trace_cnt, 0, sizeof(u32)));
   return SCS_OK;
 }
 
@@ -509,7 +573,7 @@ extern "C" void scs_engine_destroy(scs_engine* E) {
   void* ptrs[] = {E->d_slots, E->d_witness, E->d_wave, E->d_pool_seq, E->d_pool_data,
                   E->d_pool_pos, E->d_glob, E->d_stamp, E->d_dist, E->d_F, E->d_Fn, E->d_locid,
                   E->d_korder, E->d_ldeg, E->d_bsum, E->d_cnt, E->d_admit, E->d_prepbytes,
-                  E->d_arena};
+                  E->d_arena, E->d_model, E->d_trace, E->d_trace_cnt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (E->h_cnt) (void)hipHostFree(E->h_cnt);
@@ -645,7 +709,9 @@ static int prep_query(scs_engine* E, u32 si, u32 kl0, u32 ubD) {
   Q.limit_doms = hs.pq.ld;
   const scs_params& p = E->p;
   Q.flags = (p.red1 ? QF_RED1 : 0) | (p.red2 ? QF_RED2 : 0) | (p.red3 ? QF_RED3 : 0) |
-            (p.prun1 ? QF_PRUN1 : 0) | (p.prun2 ? QF_PRUN2 : 0) | QF_DOMS;
+            (p.prun1 ? QF_PRUN1 : 0) | (p.prun2 ? QF_PRUN2 : 0) | QF_DOMS |
+            (p.learned ? QF_LEARNED : 0);
+  Q.tag = (u32)hs.pq.tag;
   Q.ku = hs.ku;
   Q.K = K;
   Q.dmax = hg.dmax;

@@ -24,6 +24,7 @@
 // rules 1+2 to a fixpoint, U_d and U_nr, ustar by fixed-point connection
 // score, the set dominated by ustar, then children A, B_0.., Z.
 #include "scs_internal.h"
+#include "scs_mlp.h"
 
 #define LANE (threadIdx.x)
 #define INV 0xffffffffu
@@ -94,7 +95,10 @@ struct Lds {
   u32 cnt, cnt2, nfront, ncand, nh, nk, pad0, pad1;
   u32 D[SCS_FRAME_DOMS];
   u32 hist[SCS_HCAP + 4];   // dC histogram of R (live copy of the wave's state)
-  u32 chist[SCS_HCAP + 4];  // scratch histogram of C degrees (U_nr)
+  union {                   // never live at the same time within a node
+    u32 chist[SCS_HCAP + 4];  // scratch histogram of C degrees (U_nr)
+    u16 h1s[16 * MLP_H];      // MLP hidden-layer tile (bf16)
+  };
 };
 
 // ---------------------------------------------------------------------------
@@ -125,7 +129,13 @@ struct W {
   const u32* off;
   const u32* adj;
   const u32* dist;
-  u32 K, l, h, flags, limit_doms, dmax, ku;
+  const u32* core;
+  u32 K, l, h, flags, limit_doms, dmax, ku, tag;
+  // learned ordering and search traces
+  const MlpDev* model;
+  u32* trace;
+  u32* trace_cnt;
+  u32 trace_cap, trace_stride;
   // wave-uniform counters
   u32 nC, nR, ntrail, nframes, nsav;
   u64 edges;
@@ -154,6 +164,11 @@ __device__ void bind(W& w, const EngineArgs& A, u32 wave, Lds* L) {
   w.L = L;
   w.edges = 0;
   w.nsav = 0;
+  w.model = A.model;
+  w.trace = A.trace;
+  w.trace_cnt = A.trace_cnt;
+  w.trace_cap = A.trace_cap;
+  w.trace_stride = A.trace_stride ? A.trace_stride : 1;
 }
 
 __device__ void bind_query(W& w, QuerySlot* Q) {
@@ -161,6 +176,8 @@ __device__ void bind_query(W& w, QuerySlot* Q) {
   w.off = Q->off;
   w.adj = Q->adj;
   w.dist = Q->dist;
+  w.core = Q->core;
+  w.tag = Q->tag;
   w.K = Q->K;
   w.l = Q->l;
   w.h = Q->h;
@@ -601,7 +618,65 @@ struct NodeOut {
   u32 klc;  // kl at the end of the closure
 };
 
-__device__ NodeOut process(W& w, u32 from, u32 kl_seen) {
+__device__ __forceinline__ MlpFeat cand_features(const W& w, u32 x, u32 Cs, u32 kl) {
+  MlpFeat f;
+  f.dC = w.dC[x];
+  f.dR = w.dR[x];
+  f.dist = w.dist[x];
+  f.core = w.core[x];
+  f.slack = (int)(f.dC + min(f.dR, w.h - Cs - 1)) - (int)kl;
+  f.cscore = (float)((double)(w.acc[x] + rterm(f.dR, w.dmax)) * (1.0 / 4294967296.0));
+  return f;
+}
+
+// Highest MLP score over the frontier, ties to the lowest id (INV if no
+// finite score).
+__device__ u32 learned_ustar(W& w, u32 nf, u32 Cs, u32 kl) {
+  float bestf = -INFINITY;
+  u32 bestx = INV;
+  for (u32 t0 = 0; t0 < nf; t0 += 16) {
+    auto feat = [&](u32 row) {
+      const u32 i = t0 + row;
+      if (i < nf) return cand_features(w, w.front[i], Cs, kl);
+      return MlpFeat{0, 0, 0, 0, 0, 0.f};
+    };
+    const float sc = mlp_tile(w.model, feat, w.L->h1s);
+    const u32 i = t0 + (LANE & 15u);
+    if (LANE < 16 && i < nf) {
+      const u32 x = w.front[i];
+      if (sc > bestf || (sc == bestf && x < bestx)) {
+        bestf = sc;
+        bestx = x;
+      }
+    }
+  }
+  float gb = bestf;
+  for (int o = 32; o > 0; o >>= 1) gb = fmaxf(gb, __shfl_xor(gb, o));
+  return wmin((bestx != INV && bestf == gb) ? bestx : INV);
+}
+
+// Records up to 64 frontier candidates of this node as trace rows.
+__device__ void trace_node(W& w, u32 nf, u32 Cs, u32 kl) {
+  const u32 k = min(nf, 64u);
+  u32 base = 0;
+  if (LANE == 0) base = atomicAdd(w.trace_cnt, k);
+  base = bcast(base);
+  if (LANE < k && base + LANE < w.trace_cap) {
+    const u32 x = w.front[LANE];
+    const MlpFeat f = cand_features(w, x, Cs, kl);
+    u32* r = w.trace + (u64)(base + LANE) * 8;
+    r[0] = w.tag;
+    r[1] = w.Q->orig[x];
+    r[2] = f.dC;
+    r[3] = f.dR;
+    r[4] = f.dist;
+    r[5] = f.core;
+    r[6] = (u32)f.slack;
+    r[7] = __float_as_uint(f.cscore);
+  }
+}
+
+__device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno) {
   Lds* L = w.L;
   NodeOut o;
   o.branch = 0;
@@ -708,7 +783,13 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen) {
     }
   }
   const u64 gbest = wmax64(best);
-  const u32 us = wmin(best == gbest ? bid : INV);
+  u32 us = wmin(best == gbest ? bid : INV);
+  if (nf > 0 && w.trace && (nodeno % w.trace_stride) == 0) trace_node(w, nf, Cs, kl);
+  if (nf > 0 && us != INV && (w.flags & QF_LEARNED) && w.model) {
+    // learned order: any frontier vertex is a valid branching vertex
+    const u32 ul = learned_ustar(w, nf, Cs, kl);
+    if (ul != INV) us = ul;
+  }
   if (nf == 0 || us == INV) {
     for (u32 i = LANE; i < nf; i += 64) w.acc[w.front[i]] = 0;
     wsync();
@@ -1068,7 +1149,7 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
       finish_task(w, A, qslot);
       continue;
     }
-    NodeOut o = process(w, from, kl_seen);
+    NodeOut o = process(w, from, kl_seen, w.trace ? aload64(&Q->nodes) + pend : 0ull);
     pend++;
     if (pend >= A.node_flush) flush_counters(w, A, pend);
     if (o.branch) {

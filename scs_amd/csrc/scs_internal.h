@@ -32,6 +32,7 @@ typedef int64_t i64;
 #define QF_PRUN1 8u
 #define QF_PRUN2 16u
 #define QF_DOMS 32u
+#define QF_LEARNED 64u
 
 // Per-query slot, device resident.  Written by the host at admission (while
 // no kernel touches the slot) and by search waves through atomics.
@@ -55,7 +56,8 @@ struct QuerySlot {
   const u32* orig; // local -> original vertex id
   const u32* core; // core numbers (MLP feature)
   u32* witness;    // h entries, original ids
-  u32 pad[2];
+  u32 tag;         // low 32 bits of the caller's tag (search traces)
+  u32 pad;
 };
 
 // Depth-first frame: one per branching node on the current path.
@@ -116,7 +118,14 @@ struct EngineGlobals {
   u32 pad[5];
 };
 
+struct MlpDev;
+
 struct EngineArgs {
+  const MlpDev* model;  // branching MLP (queries with QF_LEARNED)
+  u32* trace;           // search-trace rows (8 words each), or null
+  u32* trace_cnt;
+  u32 trace_cap;
+  u32 trace_stride;
   QuerySlot* slots;
   u32 nslots;
   u8* wave_mem;

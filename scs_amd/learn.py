@@ -1,0 +1,145 @@
+"""Learned branching order: traces, training and evaluation.
+
+1. Traces.  The engine runs a stream of training queries with tracing on;
+   every stride-th search node records up to 64 of its branching
+   candidates with six features (dC, dR, dist(q, v), core(v),
+   slack = dC + min(dR, h-|C|-1) - kl, and the plain connection score).
+2. Labels.  A candidate is positive when it belongs to the best community
+   H the search found for its query (the witness), i.e. choosing it leads
+   towards the optimum.
+3. Training.  The 6-64-64-1 MLP is trained on the GPU by the library's own
+   forward/backward/Adam kernels (scs_mlp_train) with class-balanced binary
+   cross-entropy.
+4. Evaluation.  The same query stream is run with the plain connection-score
+   rule and with the learned order; both return the same optimum (the order
+   never prunes), and the node counts and times are compared.
+
+Usage:
+  python -m scs_amd.learn --config C1 --out models/branching_c1.npz
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import configs, graphgen as gg, scs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def training_queries(name: str, core: np.ndarray, seeds=range(100, 200), per_seed=4):
+    """Query streams drawn with seeds 100..199 (disjoint from the evaluation
+    stream, whose seed is graph seed + 1000)."""
+    c = configs.CONFIGS[name]
+    qp = dict(c["queries"])
+    qp["count"] = per_seed
+    out = []
+    for s in seeds:
+        out.extend(gg.make_queries(core, seed=s, **qp))
+    return [(q.l, q.h, q.q, q.heu, q.limit_doms) for q in out]
+
+
+def collect(G, queries, budget, stride=2, cap=2_000_000, **engine_kw):
+    E = scs.Engine(G, node_budget=budget, **engine_kw)
+    E.trace(cap, stride)
+    res = E.run(queries, tags=list(range(len(queries))), progress=10.0)
+    rows = E.trace_read()
+    E.close()
+    wit = {r.tag: set(int(x) for x in r.witness) for r in res}
+    X = np.zeros((len(rows), 6), np.float32)
+    X[:, 0] = rows[:, 2]
+    X[:, 1] = rows[:, 3]
+    X[:, 2] = rows[:, 4]
+    X[:, 3] = rows[:, 5]
+    X[:, 4] = rows[:, 6].view(np.int32)
+    X[:, 5] = rows[:, 7].view(np.float32)
+    y = np.array([1.0 if int(v) in wit.get(int(t), ()) else 0.0 for t, v in rows[:, :2]],
+                 np.float32)
+    return X, y, res
+
+
+def train(X, y, epochs=20, batch=2048, lr=2e-3, seed=0):
+    m = scs.model_init(seed)
+    pos = max(1.0, float(y.sum()))
+    pw = float(min(50.0, (len(y) - pos) / pos))
+    loss = scs.mlp_train(m, X, y, epochs=epochs, batch=min(batch, len(y)), lr=lr,
+                         pos_weight=pw, seed=seed)
+    return m, loss, pw
+
+
+def evaluate(G, queries, model, budget, **engine_kw):
+    """Runs the stream with the plain rule and with the learned order."""
+    out = {}
+    for mode in ("plain", "learned"):
+        E = scs.Engine(G, node_budget=budget, **engine_kw)
+        if mode == "learned":
+            E.set_model(model, True)
+        t0 = time.perf_counter()
+        res = E.run(queries, progress=10.0)
+        dt = time.perf_counter() - t0
+        print(f"[learn] {mode}: {dt:.1f} s", file=sys.stderr, flush=True)
+        st = E.stats()
+        E.close()
+        proven = sum(r.status in ("OK", "HEU_OPT", "INFEASIBLE") for r in res)
+        out[mode] = {"seconds": dt, "nodes": int(st["nodes"]), "proven": proven,
+                     "kl": [r.kl for r in res], "status": [r.status for r in res],
+                     "nodes_per_query": [int(r.nodes) for r in res]}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="C1")
+    ap.add_argument("--per-seed", type=int, default=4)
+    ap.add_argument("--budget", type=int, default=20000)
+    ap.add_argument("--stride", type=int, default=2)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--eval-queries", type=int, default=100)
+    ap.add_argument("--out", default=os.path.join(ROOT, "models", "branching_c1.npz"))
+    ap.add_argument("--report", default=None)
+    args = ap.parse_args()
+
+    g = configs.graph(args.config)
+    G = scs.Graph.from_csr(g.off, g.adj)
+    core = G.core().astype(np.int64)
+    tq = training_queries(args.config, core, per_seed=args.per_seed)
+    t0 = time.perf_counter()
+    X, y, _ = collect(G, tq, args.budget, stride=args.stride, limit_queries=256)
+    t_col = time.perf_counter() - t0
+    print(f"[learn] {len(tq)} training queries -> {len(y)} rows, {int(y.sum())} positive "
+          f"({t_col:.1f} s)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    m, loss, pw = train(X, y, epochs=args.epochs)
+    t_tr = time.perf_counter() - t0
+    print(f"[learn] trained: loss {loss:.4f} pos_weight {pw:.1f} ({t_tr:.1f} s)", file=sys.stderr,
+          flush=True)
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    m.save(args.out)
+    eq = [(q.l, q.h, q.q, q.heu, q.limit_doms)
+          for q in configs.queries(args.config, core, count=args.eval_queries)]
+    ev = evaluate(G, eq, m, args.budget, limit_queries=256)
+    same = all(a == b for a, b, s1, s2 in zip(ev["plain"]["kl"], ev["learned"]["kl"],
+                                                 ev["plain"]["status"], ev["learned"]["status"])
+               if s1 != "CAPPED" and s2 != "CAPPED")
+    rep = {"config": args.config, "train_queries": len(tq), "rows": int(len(y)),
+           "positives": int(y.sum()), "loss": loss, "collect_s": t_col, "train_s": t_tr,
+           "eval_queries": len(eq), "budget": args.budget,
+           "plain": {k: ev["plain"][k] for k in ("seconds", "nodes", "proven")},
+           "learned": {k: ev["learned"][k] for k in ("seconds", "nodes", "proven")},
+           "optimum_equal_where_both_complete": same,
+           "kl_sum_plain": int(sum(ev["plain"]["kl"])), "kl_sum_learned": int(sum(ev["learned"]["kl"]))}
+    print(json.dumps(rep), flush=True)
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(rep, f, indent=1)
+    G.close()
+
+
+if __name__ == "__main__":
+    main()

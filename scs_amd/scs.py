@@ -63,6 +63,38 @@ class Stats(C.Structure):
                 ("kcap", C.c_uint32), ("edges", C.c_uint64), ("core_us", C.c_double)]
 
 
+class Model(C.Structure):
+    """scs_model: FP32 master weights of the branching MLP (include/scs.h)."""
+    _fields_ = [("w1", (C.c_float * 64) * 8), ("b1", C.c_float * 64),
+                ("w2", (C.c_float * 64) * 64), ("b2", C.c_float * 64), ("w3", C.c_float * 64),
+                ("b3", C.c_float), ("mean", C.c_float * 8), ("scale", C.c_float * 8),
+                ("pad", C.c_float * 7)]
+
+    def to_arrays(self):
+        return {f: np.ctypeslib.as_array(getattr(self, f)).copy() if f != "b3" else np.float32(self.b3)
+                for f, _ in self._fields_ if f != "pad"}
+
+    @classmethod
+    def from_arrays(cls, d):
+        m = cls()
+        for f, _ in cls._fields_:
+            if f == "pad":
+                continue
+            if f == "b3":
+                m.b3 = float(d["b3"])
+            else:
+                np.ctypeslib.as_array(getattr(m, f))[...] = np.asarray(d[f], np.float32)
+        return m
+
+    def save(self, path):
+        np.savez(path, **self.to_arrays())
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as z:
+            return cls.from_arrays({k: z[k] for k in z.files})
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -94,6 +126,18 @@ def lib():
         L.scs_engine_stats.argtypes = [P, C.POINTER(Stats)]
         L.scs_strerror.restype = C.c_char_p
         L.scs_strerror.argtypes = [C.c_int]
+        f32p = np.ctypeslib.ndpointer(np.float32, flags="C")
+        L.scs_model_init.argtypes = [C.POINTER(Model), C.c_uint64]
+        L.scs_mlp_grad.argtypes = [C.POINTER(Model), f32p, f32p, C.c_uint32, C.c_float,
+                                   C.POINTER(Model), C.POINTER(C.c_float)]
+        L.scs_mlp_train.argtypes = [C.POINTER(Model), f32p, f32p, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_float, C.c_float, C.c_uint64,
+                                    C.POINTER(C.c_float)]
+        L.scs_mlp_infer.argtypes = [C.POINTER(Model), f32p, C.c_uint32, f32p]
+        L.scs_engine_set_model.argtypes = [P, C.POINTER(Model), C.c_uint32]
+        L.scs_engine_trace.argtypes = [P, C.c_uint32, C.c_uint32]
+        L.scs_engine_trace_read.argtypes = [P, np.ctypeslib.ndpointer(np.uint32, flags="C"),
+                                            C.c_uint32, C.POINTER(C.c_uint32)]
         _LIB = L
     return _LIB
 
@@ -182,12 +226,48 @@ def _commas(x: int) -> str:
     return f"{x:,}"
 
 
+def model_init(seed=0):
+    m = Model()
+    _check(lib().scs_model_init(C.byref(m), seed), "scs_model_init")
+    return m
+
+
+def mlp_grad(model, X, y, pos_weight=1.0):
+    X = np.ascontiguousarray(X, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    g = Model()
+    loss = C.c_float()
+    _check(lib().scs_mlp_grad(C.byref(model), X, y, len(y), pos_weight, C.byref(g),
+                              C.byref(loss)), "scs_mlp_grad")
+    return g, loss.value
+
+
+def mlp_train(model, X, y, epochs=10, batch=1024, lr=1e-3, pos_weight=1.0, seed=0):
+    X = np.ascontiguousarray(X, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    loss = C.c_float()
+    _check(lib().scs_mlp_train(C.byref(model), X, y, len(y), epochs, batch, lr, pos_weight,
+                               seed, C.byref(loss)), "scs_mlp_train")
+    return loss.value
+
+
+def mlp_infer(model, X):
+    X = np.ascontiguousarray(X, np.float32)
+    out = np.zeros(len(X), np.float32)
+    _check(lib().scs_mlp_infer(C.byref(model), X, len(X), out), "scs_mlp_infer")
+    return out
+
+
 class Engine:
     """Search engine over one graph (see scs_params in include/scs.h)."""
 
     def __init__(self, graph: Graph, **kw):
         p = Params()
         lib().scs_params_default(C.byref(p))
+        model = kw.pop("model", None)
+        if model is not None:
+            self._model_ref = model
+            p.model = C.cast(C.pointer(model), C.c_void_p)
         for k, v in kw.items():
             if not hasattr(p, k):
                 raise TypeError(f"unknown engine parameter {k}")
@@ -224,6 +304,21 @@ class Engine:
 
     def drain(self):
         return _check(lib().scs_drain(self._h), "scs_drain")
+
+    def set_model(self, model, learned=True):
+        _check(lib().scs_engine_set_model(self._h, C.byref(model) if model is not None else None,
+                                          1 if learned else 0), "scs_engine_set_model")
+
+    def trace(self, capacity_rows, stride=1):
+        _check(lib().scs_engine_trace(self._h, capacity_rows, stride), "scs_engine_trace")
+        self._trace_cap = capacity_rows
+
+    def trace_read(self):
+        cap = getattr(self, "_trace_cap", 0)
+        rows = np.zeros((max(cap, 1), 8), np.uint32)
+        n = C.c_uint32()
+        _check(lib().scs_engine_trace_read(self._h, rows, cap, C.byref(n)), "trace_read")
+        return rows[:n.value].copy()
 
     def stats(self):
         s = Stats()

@@ -1,0 +1,131 @@
+"""Branching MLP on the GPU: training gradients against the NumPy
+restatement, BF16 MFMA inference against a BF16-rounded reference, and the
+learned order leaving every optimum unchanged."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.mlp_ref import forward_backward, infer_bf16
+from scs_amd import scs
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def random_data(n, seed):
+    rng = np.random.default_rng(seed)
+    X = np.stack([rng.integers(0, 20, n), rng.integers(0, 200, n), rng.integers(1, 6, n),
+                  rng.integers(1, 60, n), rng.integers(1, 30, n), rng.random(n) * 4],
+                 axis=1).astype(np.float32)
+    y = (rng.random(n) < 0.3).astype(np.float32)
+    return X, y
+
+
+def model_dict(m):
+    return m.to_arrays()
+
+
+def test_gradients_match_reference():
+    m = scs.model_init(3)
+    d = model_dict(m)
+    d["mean"][:6] = [5, 50, 2, 20, 10, 15]
+    d["scale"][:6] = [0.2, 0.02, 0.5, 0.05, 0.1, 0.08]
+    m = scs.Model.from_arrays(d)
+    X, y = random_data(300, 1)
+    g, loss = scs.mlp_grad(m, X, y, pos_weight=2.5)
+    rl, rg = forward_backward(model_dict(m), X, y, pos_weight=2.5)
+    assert abs(loss - rl) <= 1e-4 * max(1.0, abs(rl))
+    gd = g.to_arrays()
+    for k in ("w1", "b1", "w2", "b2", "w3"):
+        ref = rg[k][:6] if k == "w1" else rg[k]
+        got = gd[k][:6] if k == "w1" else gd[k]
+        tol = 1e-4 * max(1.0, float(np.abs(ref).max()))
+        assert np.allclose(got, ref, rtol=1e-3, atol=tol), k
+    assert abs(float(gd["b3"]) - float(rg["b3"])) < 1e-4
+
+
+def test_mfma_inference_matches_bf16_reference():
+    m = scs.model_init(5)
+    d = model_dict(m)
+    d["mean"][:6] = [5, 50, 2, 20, 10, 15]
+    d["scale"][:6] = [0.2, 0.02, 0.5, 0.05, 0.1, 0.08]
+    d["b1"][:] = 0.05
+    d["b2"][:] = -0.02
+    m = scs.Model.from_arrays(d)
+    X, _ = random_data(1000, 2)
+    got = scs.mlp_infer(m, X)
+    ref = infer_bf16(model_dict(m), X)
+    scale = max(1.0, float(np.abs(ref).max()))
+    assert np.allclose(got, ref, rtol=2e-2, atol=2e-2 * scale)
+    # ordering agrees on clearly separated scores
+    order = np.argsort(-ref)[:50]
+    assert np.corrcoef(got[order], ref[order])[0, 1] > 0.99
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(0)
+    X, _ = random_data(8192, 4)
+    y = ((X[:, 0] * 3 + X[:, 3] - X[:, 2] * 5) > 40).astype(np.float32)
+    m0 = scs.model_init(1)
+    _, l0 = scs.mlp_grad(m0, X, y)
+    m = scs.model_init(1)
+    l1 = scs.mlp_train(m, X, y, epochs=15, batch=1024, lr=3e-3)
+    assert l1 < 0.5 * l0
+    acc = ((scs.mlp_infer(m, X) > 0) == (y > 0.5)).mean()
+    assert acc > 0.9
+
+
+def test_learned_order_keeps_every_optimum():
+    gold = json.load(open(os.path.join(GOLD, "golden.json")))
+    from scs_amd import configs
+    g = configs.graph("C1")
+    G = scs.Graph.from_csr(g.off, g.adj)
+    m = scs.model_init(9)
+    d = model_dict(m)
+    d["mean"][:6] = [3, 30, 2, 20, 10, 10]
+    d["scale"][:6] = [0.3, 0.03, 0.5, 0.05, 0.1, 0.1]
+    m = scs.Model.from_arrays(d)
+    rows = gold["C1"] + gold["hepPH_SCS"][:0]
+    qs = [tuple(r["query"]) for r in rows]
+    # an untrained order can make the tree far larger: bound it; a complete
+    # search must return the optimum, a capped one a lower bound of it
+    E = scs.Engine(G, limit_queries=16, node_budget=400_000)
+    E.set_model(m, True)
+    res = E.run(qs)
+    E.close()
+    complete = 0
+    for r, row in zip(res, rows):
+        if r.status == "CAPPED":
+            assert r.heu_kl <= r.kl <= row["kl"]
+        else:
+            complete += 1
+            assert r.kl == row["kl"] and r.status in ("OK", "HEU_OPT")
+    assert complete >= 1
+    G.close()
+
+
+def test_trained_model_gain_over_plain_rule():
+    """The committed model (trained by scs_amd.learn on traces of training
+    queries drawn with seeds 100..199) against the plain connection score on
+    unseen C1 queries with the same node budget: identical optima where both
+    searches complete, and at least as good a total of found kl."""
+    from scs_amd import configs, learn
+    path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "models", "branching_c1.npz")
+    m = scs.Model.load(path)
+    g = configs.graph("C1")
+    G = scs.Graph.from_csr(g.off, g.adj)
+    core = G.core().astype(np.int64)
+    qs = [(q.l, q.h, q.q, q.heu, q.limit_doms) for q in configs.queries("C1", core, count=60)]
+    ev = learn.evaluate(G, qs, m, 20000, limit_queries=64)
+    G.close()
+    p, l = ev["plain"], ev["learned"]
+    for kp, kl_, sp, sl in zip(p["kl"], l["kl"], p["status"], l["status"]):
+        if sp != "CAPPED" and sl != "CAPPED":
+            assert kp == kl_
+    print("plain kl sum", sum(p["kl"]), "learned kl sum", sum(l["kl"]),
+          "proven", p["proven"], l["proven"])
+    assert sum(l["kl"]) >= sum(p["kl"])
