@@ -45,7 +45,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--queries", type=int, default=2000)
-    ap.add_argument("--budget", type=int, default=20000, help="node budget per query")
+    ap.add_argument("--budget", type=int, default=300, help="node budget per query")
     ap.add_argument("--waves-per-cu", type=int, default=16)
     ap.add_argument("--limit-queries", type=int, default=512)
     ap.add_argument("--slice-us", type=int, default=5000)

@@ -16,6 +16,7 @@
 #include <thread>
 #include <vector>
 
+#include "scs_arena.h"
 #include "scs_host.h"
 #include "scs_internal.h"
 #include "scs_mlp.h"
@@ -260,53 +261,6 @@ extern "C" void scs_graph_destroy(scs_graph* g) {
 }
 
 // ---------------------------------------------------------------------------
-// Device arena for per-query local graphs (first fit, 256-byte granules).
-// ---------------------------------------------------------------------------
-struct Arena {
-  u8* base = nullptr;
-  u64 size = 0;
-  std::map<u64, u64> free_;  // offset -> length
-  void init(u8* b, u64 s) {
-    base = b;
-    size = s;
-    free_.clear();
-    free_[0] = s;
-  }
-  bool alloc(u64 bytes, u64* off) {
-    bytes = (bytes + 255) & ~255ull;
-    if (bytes == 0) bytes = 256;
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= bytes) {
-        *off = it->first;
-        u64 rest = it->second - bytes;
-        u64 noff = it->first + bytes;
-        free_.erase(it);
-        if (rest) free_[noff] = rest;
-        return true;
-      }
-    }
-    return false;
-  }
-  void release(u64 off, u64 bytes) {
-    bytes = (bytes + 255) & ~255ull;
-    if (bytes == 0) bytes = 256;
-    auto it = free_.emplace(off, bytes).first;
-    auto nx = std::next(it);
-    if (nx != free_.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free_.erase(nx);
-    }
-    if (it != free_.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-};
-
-// ---------------------------------------------------------------------------
 // Engine
 // ---------------------------------------------------------------------------
 struct PendingQuery {
@@ -318,6 +272,7 @@ struct PendingQuery {
 struct HostSlot {
   bool used = false;
   PendingQuery pq;
+  std::vector<u32> seed_witness;  // heuristic H, copied into the slot at admission
   u32 kl0 = 0, ku = 0, K = 0;
   u64 a_off = 0, a_len = 0, b_off = 0, b_len = 0;
 };
@@ -325,7 +280,8 @@ struct HostSlot {
 struct scs_engine {
   scs_graph* g = nullptr;
   scs_params p{};
-  hipStream_t s = nullptr;
+  hipStream_t s = nullptr;      // search epochs, slot writes, admission
+  hipStream_t ps = nullptr;     // per-query preprocessing (overlaps epochs)
   u32 nwaves = 0, kcap = 0;
   // device
   QuerySlot* d_slots = nullptr;
@@ -347,7 +303,7 @@ struct scs_engine {
   u32* d_trace = nullptr;
   u32* d_trace_cnt = nullptr;
   u32 trace_cap = 0;
-  Arena arena;
+  scs::Arena arena;
   u32 stamp_id = 0;
   u32* h_cnt = nullptr;  // pinned
   hipEvent_t ev0 = nullptr, ev1 = nullptr, pe0 = nullptr, pe1 = nullptr;
@@ -433,6 +389,7 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
     return rc;
   };
   if (hipStreamCreate(&E->s) != hipSuccess) return fail(SCS_EHIP);
+  if (hipStreamCreate(&E->ps) != hipSuccess) return fail(SCS_EHIP);
   int dev = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
@@ -570,6 +527,7 @@ extern "C" int scs_engine_trace_read(scs_engine* E, uint32_t* rows, uint32_t max
 extern "C" void scs_engine_destroy(scs_engine* E) {
   if (!E) return;
   if (E->s) (void)hipStreamSynchronize(E->s);
+  if (E->ps) (void)hipStreamSynchronize(E->ps);
   void* ptrs[] = {E->d_slots, E->d_witness, E->d_wave, E->d_pool_seq, E->d_pool_data,
                   E->d_pool_pos, E->d_glob, E->d_stamp, E->d_dist, E->d_F, E->d_Fn, E->d_locid,
                   E->d_korder, E->d_ldeg, E->d_bsum, E->d_cnt, E->d_admit, E->d_prepbytes,
@@ -580,6 +538,7 @@ extern "C" void scs_engine_destroy(scs_engine* E) {
   for (hipEvent_t ev : {E->ev0, E->ev1, E->pe0, E->pe1})
     if (ev) (void)hipEventDestroy(ev);
   if (E->s) (void)hipStreamDestroy(E->s);
+  if (E->ps) (void)hipStreamDestroy(E->ps);
   delete E;
 }
 
@@ -643,7 +602,7 @@ static int prep_query(scs_engine* E, u32 si, u32 kl0, u32 ubD) {
   HostSlot& hs = E->slots[si];
   const scs::HostGraph& hg = E->g->hg;
   const u32 n = hg.n;
-  hipStream_t s = E->s;
+  hipStream_t s = E->ps;
   if (++E->stamp_id == 0) {
     HIPCHK(hipMemsetAsync(E->d_stamp, 0, sizeof(u32) * n, s));
     E->stamp_id = 1;
@@ -673,9 +632,12 @@ static int prep_query(scs_engine* E, u32 si, u32 kl0, u32 ubD) {
   HIPCHK(scs_launch_kept_order(n, E->d_stamp, id, E->d_locid, E->d_korder, s));
   HIPCHK(scs_launch_local_degree(E->g->d_off, E->g->d_adj, E->d_stamp, id, E->d_korder, K,
                                  E->d_ldeg, s));
-  // block A: loff (K+1) | ldist (K) | lcore (K) | lorig (K)
-  hs.a_len = sizeof(u32) * (4ull * K + 1);
-  if (!E->arena.alloc(hs.a_len, &hs.a_off)) return SCS_EAGAIN;
+  // block A: loff (K+1) | ldist (K) | lcore (K) | lorig (K).  a_len/b_len
+  // are set only once the allocation succeeded: release_slot frees what
+  // they describe.
+  const u64 a_len = sizeof(u32) * (4ull * K + 1);
+  if (!E->arena.alloc(a_len, &hs.a_off)) return SCS_EAGAIN;
+  hs.a_len = a_len;
   u32* A = (u32*)(E->d_arena + hs.a_off);
   u32 *loff = A, *ldist = A + K + 1, *lcore = ldist + K, *lorig = lcore + K;
   HIPCHK(scs_launch_scan_u32(E->d_ldeg, K, E->d_bsum, loff, E->d_cnt + 3, s));
@@ -683,12 +645,13 @@ static int prep_query(scs_engine* E, u32 si, u32 kl0, u32 ubD) {
   HIPCHK(hipMemcpyAsync(E->h_cnt + 3, E->d_cnt + 3, sizeof(u32), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const u32 nnzK = E->h_cnt[3];
-  hs.b_len = sizeof(u32) * (u64)std::max<u32>(nnzK, 1);
-  if (!E->arena.alloc(hs.b_len, &hs.b_off)) {
+  const u64 b_len = sizeof(u32) * (u64)std::max<u32>(nnzK, 1);
+  if (!E->arena.alloc(b_len, &hs.b_off)) {
     E->arena.release(hs.a_off, hs.a_len);
     hs.a_len = 0;
     return SCS_EAGAIN;
   }
+  hs.b_len = b_len;
   u32* ladj = (u32*)(E->d_arena + hs.b_off);
   HIPCHK(scs_launch_local_fill(E->g->d_off, E->g->d_adj, E->d_stamp, id, E->d_locid, E->d_korder,
                                K, loff, E->d_dist, E->g->d_core, ladj, ldist, lcore, lorig, s));
@@ -730,8 +693,10 @@ static int prep_query(scs_engine* E, u32 si, u32 kl0, u32 ubD) {
 
 static void release_slot(scs_engine* E, u32 si) {
   HostSlot& hs = E->slots[si];
-  if (hs.a_len) E->arena.release(hs.a_off, hs.a_len);
-  if (hs.b_len) E->arena.release(hs.b_off, hs.b_len);
+  bool ok = true;
+  if (hs.a_len) ok &= E->arena.release(hs.a_off, hs.a_len);
+  if (hs.b_len) ok &= E->arena.release(hs.b_off, hs.b_len);
+  if (!ok) fprintf(stderr, "scs: arena release of slot %u refused (range not allocated)\n", si);
   hs.a_len = hs.b_len = 0;
   hs.used = false;
 }
@@ -772,9 +737,40 @@ static int collect(scs_engine* E) {
   return SCS_OK;
 }
 
+static int launch_epoch(scs_engine* E) {
+  EngineGlobals gl;
+  HIPCHK(hipMemcpy(&gl, E->d_glob, sizeof(gl), hipMemcpyDeviceToHost));
+  gl.idle = 0;
+  gl.active_q = E->active;
+  HIPCHK(hipMemcpyAsync(E->d_glob, &gl, sizeof(gl), hipMemcpyHostToDevice, E->s));
+  HIPCHK(hipEventRecord(E->ev0, E->s));
+  HIPCHK(scs_launch_epoch(&E->args, E->s));
+  HIPCHK(hipEventRecord(E->ev1, E->s));
+  return SCS_OK;
+}
+
+static int finish_epoch(scs_engine* E) {
+  HIPCHK(hipEventSynchronize(E->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+  E->st.engine_us += ms * 1000.0;
+  E->st.epochs++;
+  return collect(E);
+}
+
+// One serving step.  The search epoch of the queries already admitted runs
+// on the engine stream while newly queued queries get their heuristic (host
+// threads) and GPU preprocessing (second stream); they are written into
+// their slots and their root tasks pushed once the epoch has ended, so they
+// search from the next step on.
 extern "C" int scs_step(scs_engine* E) {
   if (!E) return SCS_EINVAL;
   const scs::HostGraph& hg = E->g->hg;
+  const bool running = E->active > 0;
+  if (running) {
+    int rc = launch_epoch(E);
+    if (rc) return rc;
+  }
   // 1. take queued queries for free slots
   std::vector<u32> free_slots;
   for (u32 si = 0; si < E->p.limit_queries; si++)
@@ -790,7 +786,7 @@ extern "C" int scs_step(scs_engine* E) {
   // 2. heuristic lower bounds on host threads
   std::vector<u32> kl0(batch.size(), 0);
   std::vector<std::vector<u32>> H(batch.size());
-  {
+  if (!batch.empty()) {
     double t0 = now_us();
     std::atomic<size_t> next{0};
     auto worker = [&](u32 t) {
@@ -800,14 +796,14 @@ extern "C" int scs_step(scs_engine* E) {
         if (pq.heu && pq.l <= pq.h) kl0[i] = scs::cssc_heu(hg, pq.l, pq.h, pq.q, E->heu_work[t], H[i]);
       }
     };
-    u32 nt = std::min<u32>(E->p.heuristic_threads, (u32)std::max<size_t>(batch.size(), 1));
+    u32 nt = std::min<u32>(E->p.heuristic_threads, (u32)batch.size());
     std::vector<std::thread> th;
     for (u32 t = 1; t < nt; t++) th.emplace_back(worker, t);
     worker(0);
     for (auto& t : th) t.join();
     E->st.heu_us += now_us() - t0;
   }
-  // 3. admission: immediate answers or GPU preprocessing
+  // 3. immediate answers or GPU preprocessing (second stream)
   std::vector<u32> admitted;
   size_t fs = 0;
   for (size_t i = 0; i < batch.size(); i++) {
@@ -833,6 +829,7 @@ extern "C" int scs_step(scs_engine* E) {
     const u32 si = free_slots[fs];
     HostSlot& hs = E->slots[si];
     hs.used = true;
+    hs.a_len = hs.b_len = 0;
     hs.pq = pq;
     hs.kl0 = kl0[i];
     hs.ku = ku;
@@ -845,28 +842,47 @@ extern "C" int scs_step(scs_engine* E) {
     }
     if (rc == SCS_EAGAIN) {
       release_slot(E, si);
+      if (E->arena.empty()) {
+        // the local graph does not fit even in the empty arena
+        push_result(E, pq, kl0[i], kl0[i], ku, SCS_ST_REJECTED, hs.K, 0, H[i].data(),
+                    (u32)H[i].size());
+        continue;
+      }
       std::lock_guard<std::mutex> lk(E->qmu);
       E->queue.push_front(pq);
       continue;
     }
     if (rc) return rc;
-    // heuristic witness seeds the slot (only trusted when kl0 > 0)
     QuerySlot& Q = E->h_slots[si];
     if (kl0[i] > 0 && !H[i].empty() && H[i].size() <= SCS_MAX_H) {
       Q.wsize = (u32)H[i].size();
       Q.wkl = kl0[i];
-      HIPCHK(hipMemcpyAsync(Q.witness, H[i].data(), sizeof(u32) * H[i].size(),
-                            hipMemcpyHostToDevice, E->s));
+      hs.seed_witness = H[i];
+    } else {
+      hs.seed_witness.clear();
     }
-    HIPCHK(hipMemcpyAsync(&E->d_slots[si], &Q, sizeof(QuerySlot), hipMemcpyHostToDevice, E->s));
     admitted.push_back(si);
     fs++;
   }
+  // 4. end of the epoch: results of finished queries
+  if (running) {
+    int rc = finish_epoch(E);
+    if (rc) return rc;
+  }
+  // 5. admitted queries become visible to the search (engine stream)
   if (!admitted.empty()) {
+    for (u32 si : admitted) {
+      const HostSlot& hs = E->slots[si];
+      const QuerySlot& Q = E->h_slots[si];
+      if (!hs.seed_witness.empty())
+        HIPCHK(hipMemcpyAsync(Q.witness, hs.seed_witness.data(), sizeof(u32) * hs.seed_witness.size(),
+                              hipMemcpyHostToDevice, E->s));
+      HIPCHK(hipMemcpyAsync(&E->d_slots[si], &Q, sizeof(QuerySlot), hipMemcpyHostToDevice, E->s));
+    }
     HIPCHK(hipMemcpyAsync(E->d_admit, admitted.data(), sizeof(u32) * admitted.size(),
                           hipMemcpyHostToDevice, E->s));
-    HIPCHK(scs_launch_admit(&E->args, E->d_admit, (u32)admitted.size(), E->d_admit + E->p.limit_queries,
-                            E->s));
+    HIPCHK(scs_launch_admit(&E->args, E->d_admit, (u32)admitted.size(),
+                            E->d_admit + E->p.limit_queries, E->s));
     std::vector<u32> failed(admitted.size());
     HIPCHK(hipMemcpyAsync(failed.data(), E->d_admit + E->p.limit_queries,
                           sizeof(u32) * admitted.size(), hipMemcpyDeviceToHost, E->s));
@@ -877,24 +893,6 @@ extern "C" int scs_step(scs_engine* E) {
         return SCS_ENOMEM;
       }
     E->active += (u32)admitted.size();
-  }
-  // 4. one search epoch
-  if (E->active > 0) {
-    EngineGlobals gl;
-    HIPCHK(hipMemcpy(&gl, E->d_glob, sizeof(gl), hipMemcpyDeviceToHost));
-    gl.idle = 0;
-    gl.active_q = E->active;
-    HIPCHK(hipMemcpy(E->d_glob, &gl, sizeof(gl), hipMemcpyHostToDevice));
-    HIPCHK(hipEventRecord(E->ev0, E->s));
-    HIPCHK(scs_launch_epoch(&E->args, E->s));
-    HIPCHK(hipEventRecord(E->ev1, E->s));
-    HIPCHK(hipEventSynchronize(E->ev1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
-    E->st.engine_us += ms * 1000.0;
-    E->st.epochs++;
-    int rc = collect(E);
-    if (rc) return rc;
   }
   size_t queued;
   {

@@ -89,6 +89,11 @@ class Model(C.Structure):
     def save(self, path):
         np.savez(path, **self.to_arrays())
 
+    def save_raw(self, path):
+        """Raw scs_model bytes, the format scs_server --learned reads."""
+        with open(path, "wb") as f:
+            f.write(bytes(self))
+
     @classmethod
     def load(cls, path):
         with np.load(path, allow_pickle=False) as z:

@@ -1,0 +1,54 @@
+"""Admission under device-memory pressure.  The per-query local graphs live
+in one device arena; when it is full a query waits in the queue and is
+admitted once resident queries finish, and a query whose local graph does
+not fit even the empty arena is rejected.  With an arena holding about two
+local graphs, every golden C1 query still returns its optimum."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleGraph, ubd_host
+from scs_amd import configs, scs
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def granule(b):
+    return max(256, (b + 255) // 256 * 256)
+
+
+def test_arena_pressure_requeues_and_matches_golden():
+    rows = json.load(open(os.path.join(GOLD, "golden.json")))["C1"]
+    g = configs.graph("C1")
+    og = OracleGraph.from_csr(g.n, g.off, g.adj)
+    need = []
+    for r in rows:
+        l, h, q, heu, ld = r["query"]
+        p = og.preprocess(q, r["heu_kl"], ubd_host(r["heu_kl"], h))
+        need.append(granule(4 * (4 * p["K"] + 1)) + granule(4 * len(p["adj"])))
+    budget = 2 * max(need) + 4096
+    print("arena bytes", budget, "per-query", need, flush=True)
+    G = scs.Graph.from_csr(g.off, g.adj)
+    E = scs.Engine(G, limit_queries=16, mem_budget=budget)
+    res = E.run([tuple(r["query"]) for r in rows], progress=10.0)
+    E.close()
+    for r, row in zip(res, rows):
+        assert r.status in ("OK", "HEU_OPT"), (row["query"], r.status)
+        assert r.kl == row["kl"], row["query"]
+
+    # an arena smaller than the smallest local graph rejects every query
+    # that needs one, and the engine stays usable
+    E = scs.Engine(G, limit_queries=16, mem_budget=min(need) // 2)
+    res = E.run([tuple(r["query"]) for r in rows[:3]])
+    E.close()
+    for r, row in zip(res, rows[:3]):
+        if row["heu_kl"] == row["ku"]:
+            assert r.status == "HEU_OPT"
+        else:
+            assert r.status == "REJECTED" and r.heu_kl <= r.kl <= row["kl"]
+    G.close()
