@@ -5,12 +5,20 @@
 // can be checked against it.  Only tests/, the smoke check and bench.py's
 // CPU-baseline leg may load it; the product never links or calls it.
 //
-// Parity status of each part (how it is pinned) is listed in DESIGN.md; in
-// short: the loader, core numbers and BFS are pinned by networkx / closed
-// forms / the shipped data files, the optimum by brute force on small graphs
-// and by two independent exact solvers, the heuristic kl values by invariants
-// only ("parity unpinned" for their exact values, which never change the
-// optimum).
+// How each part is pinned (tests/test_oracle.py, tests/test_workloads.py):
+//   loader, core numbers, BFS: networkx on every shipped and generated
+//     graph, closed forms (K_n, cycles, trees), the shipped data files'
+//     edge counts;
+//   optimum: brute force over connected sets on random small graphs, the
+//     known answer of data/graph.txt (kl 3), closed forms, monotonicity in
+//     l and h, and two independent exact solvers (full rules vs no rules)
+//     agreeing on every C1 query;
+//   U_nr: equal to the brute-force optimum of its relaxation;
+//   diameter bound: hand-checked small cases;
+//   heuristic kl values: invariants only (heu kl <= optimum <= ku); their
+//     exact values are "parity unpinned" (the reference's priority-queue
+//     tie order is not reproducible without its own build), and they never
+//     change the optimum.
 //
 // Reference citations are file:line in wajidmanzoor/SCS.
 
