@@ -34,6 +34,16 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_EDGE = 4 + 7 + 6
 
 
+PHASES = ("rebuild", "rules", "bounds", "score", "doms", "move", "donate", "idle")
+
+
+def phase_shares(s0, s1):
+    """Share of the search waves' wall-clock time per phase in the timed region."""
+    d = [b - a for a, b in zip(s0["phase_ticks"], s1["phase_ticks"])]
+    tot = sum(d) or 1
+    return {k: round(v / tot, 4) for k, v in zip(PHASES, d)}
+
+
 def percentile(xs, p):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), p)) if len(xs) else 0.0
 
@@ -46,9 +56,11 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--queries", type=int, default=2000)
     ap.add_argument("--budget", type=int, default=300, help="node budget per query")
-    ap.add_argument("--waves-per-cu", type=int, default=16)
+    # defaults: the autotuner's pick for this workload (scs_amd/tuned_c2.json)
+    ap.add_argument("--waves-per-cu", type=int, default=4)
     ap.add_argument("--limit-queries", type=int, default=512)
     ap.add_argument("--slice-us", type=int, default=5000)
+    ap.add_argument("--copy-limit", type=float, default=0.5)
     ap.add_argument("--learned", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -69,7 +81,7 @@ def main():
     queries = [(q.l, q.h, q.q, q.heu, q.limit_doms) for q in qs]
     E = scs.Engine(G, node_budget=args.budget, waves_per_cu=args.waves_per_cu,
                    limit_queries=args.limit_queries, time_slice_us=args.slice_us,
-                   learned=args.learned, mem_budget=8 << 30)
+                   copy_limit=args.copy_limit, learned=args.learned, mem_budget=32 << 30)
     # load code objects outside the timed region
     E.run([queries[0]])
 
@@ -129,6 +141,8 @@ def main():
                      "bytes_per_edge": BYTES_PER_EDGE, "edges": int(edges)},
         "prep": {"device_us": round(s1["prep_us"] - s0["prep_us"], 1),
                  "algorithmic_bytes": int(s1["prep_bytes"] - s0["prep_bytes"])},
+        "search_phases": phase_shares(s0, s1),
+        "domination": {k: int(s1[k] - s0[k]) for k in ("dom_front", "dom_cand", "dom_found")},
         "heuristic_host_us": round(s1["heu_us"] - s0["heu_us"], 1),
         "core_decomposition_us": round(s1["core_us"], 1),
     }

@@ -48,6 +48,7 @@ extern "C" {
 
 #define SCS_MAX_H 1000    /* largest upper size bound h accepted */
 #define SCS_MAX_DOMS 32   /* largest limit_dominating_set accepted */
+#define SCS_MS_LEVELS 1024 /* BFS levels of one preprocessing pass (ubD < h <= 1000) */
 
 /* Result status (reference "Overtime"/"Heu" columns, main.cu:123-128,328-333,
  * 351, 454, extended with explicit per-query outcomes). */
@@ -150,7 +151,7 @@ typedef struct scs_stats {
   uint64_t donations;      /* tasks handed to the shared pool */
   uint64_t rebuilds;       /* wave state rebuilds from pooled tasks */
   double engine_us;        /* device time inside search epochs */
-  double prep_us;          /* device time inside per-query preprocessing */
+  double prep_us;          /* device time inside batched preprocessing */
   double heu_us;           /* host time inside CSSC_heu */
   uint64_t prep_bytes;     /* algorithmic bytes of preprocessing */
   uint64_t epochs;
@@ -158,6 +159,12 @@ typedef struct scs_stats {
   uint32_t kcap;
   uint64_t edges;          /* adjacency entries scanned by the search */
   double core_us;          /* device time of the core decomposition */
+  uint64_t prep_batches;   /* batched preprocessing passes (up to 64 queries each) */
+  uint64_t phase_ticks[8]; /* search wall-clock ticks (100 MHz) summed over waves:
+                              rebuild, rules, bounds, score, doms, move, donate, idle */
+  uint64_t dom_front;      /* frontier vertices summed over branching nodes */
+  uint64_t dom_cand;       /* frontier vertices tested for domination */
+  uint64_t dom_found;      /* tested vertices found dominated by ustar */
 } scs_stats;
 int scs_engine_stats(const scs_engine* e, scs_stats* s);
 
@@ -176,6 +183,13 @@ int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_t n, uint32
 int scs_prep_probe(scs_engine* e, uint32_t q, uint32_t kl0, uint32_t ubD, uint32_t* K,
                    uint64_t* nnz, uint32_t* kept, uint32_t* loff, uint32_t* dist,
                    uint32_t kept_cap, uint32_t* ladj, uint64_t adj_cap);
+/* The same for count <= 64 queries preprocessed together in one batched
+ * pass.  Query i writes K[i], nnz[i], and (when they fit) kept/dist at
+ * i*kept_cap, loff at i*(kept_cap+1) and ladj at i*adj_cap. */
+int scs_prep_probe_many(scs_engine* e, uint32_t count, const uint32_t* q, const uint32_t* kl0,
+                        const uint32_t* ubD, uint32_t* K, uint64_t* nnz, uint32_t* kept,
+                        uint32_t* loff, uint32_t* dist, uint32_t kept_cap, uint32_t* ladj,
+                        uint64_t adj_cap);
 
 /* Branching MLP (6 features -> 64 -> 64 -> 1, ReLU).  No counterpart in
  * the reference: it replaces the connection score of selectUstar

@@ -1,0 +1,93 @@
+"""Autotuner for the persistent search engine.
+
+Sweeps the engine's launch and scheduling parameters on a sample of a
+workload's query stream and keeps the configuration with the highest
+queries/s:
+  waves_per_cu   resident search waves per CU (the reference's grid shape,
+                 main.cu:543-559, chosen here by measurement)
+  limit_queries  resident query slots (the reference's limitQueries)
+  copy_limit     donation threshold: busy waves hand over work while fewer
+                 than copy_limit x waves tasks are pooled (copyLimit)
+  limit_tasks    task pool slots (limitTasks)
+  time_slice_us  length of one kernel epoch
+Every configuration runs the same queries with the same node budget, so the
+answers it returns are comparable (identical wherever a query completes).
+
+Usage:
+  python -m scs_amd.autotune --config C2 --queries 512 --budget 300 \
+      --out tuned_c2.json
+"""
+
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import configs, scs
+
+GRID = {
+    "waves_per_cu": [4, 8, 16],
+    "limit_queries": [512, 2048],
+    "copy_limit": [0.02, 0.5],
+    "time_slice_us": [5000],
+}
+
+
+def run_once(G, queries, budget, params, mem_budget):
+    E = scs.Engine(G, node_budget=budget, mem_budget=mem_budget, **params)
+    E.run([queries[0]])  # code objects and first-touch outside the timing
+    s0 = E.stats()
+    t0 = time.perf_counter()
+    res = E.run(queries)
+    wall = time.perf_counter() - t0
+    s1 = E.stats()
+    E.close()
+    proven = sum(r.status in ("OK", "HEU_OPT", "INFEASIBLE") for r in res)
+    return {"qps": len(queries) / wall, "wall_s": wall, "nodes": int(s1["nodes"] - s0["nodes"]),
+            "proven": proven, "kl_sum": int(sum(r.kl for r in res))}
+
+
+def tune(G, queries, budget, grid=None, mem_budget=32 << 30, log=sys.stderr):
+    grid = grid or GRID
+    keys = list(grid)
+    rows = []
+    for combo in itertools.product(*(grid[k] for k in keys)):
+        params = dict(zip(keys, combo))
+        r = run_once(G, queries, budget, params, mem_budget)
+        row = {**params, **r}
+        rows.append(row)
+        print(f"[tune] {json.dumps(row)}", file=log, flush=True)
+    best = max(rows, key=lambda x: x["qps"])
+    return {k: best[k] for k in keys}, rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--queries", type=int, default=512)
+    ap.add_argument("--budget", type=int, default=300)
+    ap.add_argument("--mem-gb", type=float, default=32.0)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    g = configs.graph(args.config)
+    G = scs.Graph.from_csr(g.off, g.adj)
+    core = G.core().astype(np.int64)
+    qs = [(q.l, q.h, q.q, q.heu, q.limit_doms)
+          for q in configs.queries(args.config, core, count=args.queries)]
+    best, rows = tune(G, qs, args.budget, mem_budget=int(args.mem_gb * (1 << 30)))
+    out = {"config": args.config, "queries": len(qs), "budget": args.budget, "best": best,
+           "sweep": rows}
+    print(json.dumps({"best": best}), flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(out, f, indent=1)
+    G.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -29,25 +29,29 @@ hipError_t scs_launch_core_scan(u32 n, u32 k, const u32* deg, u32* core, u32* F,
                                 u32* minrest, hipStream_t s);
 hipError_t scs_launch_core_peel(const i64* off, const int32_t* adj, u32 k, const u32* F, u32 fn,
                                 u32* deg, u32* core, u32* Fn, u32* fncount, hipStream_t s);
-hipError_t scs_launch_bfs_seed(u32 q, u32 id, u32* stamp, u32* dist, u32* F, u32* fcount,
+hipError_t scs_launch_ms_init(u32 n, const u32* core, const MsBatch* B, u64* elig, u64* vis,
+                              u64* fr, u32* flags, u32 nflags, unsigned long long* bytes,
+                              hipStream_t s);
+hipError_t scs_launch_ms_seed(const MsBatch* B, u64* vis, u64* fr, u16* dist16, hipStream_t s);
+hipError_t scs_launch_ms_level(const i64* off, const int32_t* adj, u32 n, u32 level, u64 act,
+                               const u64* elig, u64* vis, const u64* fr, u64* nf, u16* dist16,
+                               const u32* flag_prev, u32* flag_cur, unsigned long long* bytes,
                                hipStream_t s);
-hipError_t scs_launch_bfs_level(const i64* off, const int32_t* adj, const u32* core, u32 kl0,
-                                u32 id, u32 level, const u32* F, u32 fn, u32* stamp, u32* dist,
-                                u32* Fn, u32* fncount, hipStream_t s);
-hipError_t scs_launch_kept_scan(u32 n, const u32* stamp, u32 id, u32* bsum, u32* locid,
-                                u32* total, hipStream_t s);
-hipError_t scs_launch_scan_u32(const u32* in, u32 n, u32* bsum, u32* out, u32* total,
+hipError_t scs_launch_ms_count(u32 n, u32 G, const u64* vis, u64 valid, u32* gcnt,
+                               unsigned long long* bytes, hipStream_t s);
+hipError_t scs_launch_ms_bscan(u32 nb, u32 G, u32* gcnt, u32* tot, unsigned long long* bytes,
                                hipStream_t s);
-hipError_t scs_launch_kept_order(u32 n, const u32* stamp, u32 id, const u32* locid, u32* korder,
-                                 hipStream_t s);
-hipError_t scs_launch_local_degree(const i64* off, const int32_t* adj, const u32* stamp, u32 id,
-                                   const u32* korder, u32 K, u32* ldeg, hipStream_t s);
-hipError_t scs_launch_local_fill(const i64* off, const int32_t* adj, const u32* stamp, u32 id,
-                                 const u32* locid, const u32* korder, u32 K, const u32* loff,
-                                 const u32* dist, const u32* core, u32* ladj, u32* ldist,
-                                 u32* lcore, u32* lorig, hipStream_t s);
-hipError_t scs_launch_prep_bytes(const i64* off, const u32* korder, u32 K, unsigned long long* out,
-                                 hipStream_t s);
+hipError_t scs_launch_ms_scatter(u32 n, u32 G, const u64* vis, u64 valid, const u32* gcnt,
+                                 const u32* core, const u16* dist16, const MsBatch* B, u32* locid,
+                                 u32* qloc, unsigned long long* bytes, hipStream_t s);
+hipError_t scs_launch_ms_degree(const i64* off, const int32_t* adj, u32 n, const u64* vis,
+                                u64 valid, const u32* locid, const MsBatch* B,
+                                unsigned long long* bytes, hipStream_t s);
+hipError_t scs_launch_ms_scan_loff(u32 nb, const MsBatch* B, u32* tot, u64 valid,
+                                   unsigned long long* bytes, hipStream_t s);
+hipError_t scs_launch_ms_fill(const i64* off, const int32_t* adj, u32 n, const u64* vis, u64 valid,
+                              const u32* locid, const MsBatch* B, unsigned long long* bytes,
+                              hipStream_t s);
 hipError_t scs_launch_epoch(const EngineArgs* A, hipStream_t s);
 hipError_t scs_launch_admit(const EngineArgs* A, const u32* slots, u32 n, u32* failed,
                             hipStream_t s);
@@ -275,6 +279,7 @@ struct HostSlot {
   std::vector<u32> seed_witness;  // heuristic H, copied into the slot at admission
   u32 kl0 = 0, ku = 0, K = 0;
   u64 a_off = 0, a_len = 0, b_off = 0, b_len = 0;
+  u64 need = 0;  // arena bytes known to be needed when admission was refused
 };
 
 struct scs_engine {
@@ -292,9 +297,15 @@ struct scs_engine {
   u64* d_pool_pos = nullptr;
   EngineGlobals* d_glob = nullptr;
   EngineArgs args{};
-  u32 *d_stamp = nullptr, *d_dist = nullptr, *d_F = nullptr, *d_Fn = nullptr;
-  u32 *d_locid = nullptr, *d_korder = nullptr, *d_ldeg = nullptr, *d_bsum = nullptr;
-  u32* d_cnt = nullptr;
+  // batched preprocessing (bit b of a mask = query b of the batch)
+  u64 *d_elig = nullptr, *d_vis = nullptr, *d_fr = nullptr, *d_nf = nullptr;
+  u16* d_dist16 = nullptr;   // [n][64] BFS distance per query
+  u32* d_locid64 = nullptr;  // [n][64] local id per query
+  u32* d_gcnt = nullptr;     // [ceil(n/64)][64] group counts, then bases
+  u32* d_flags = nullptr;    // one per BFS level: something was found
+  u32* d_tot = nullptr;      // K[64] | nnz[64] | qloc[64]
+  MsBatch* d_batch = nullptr;
+  MsBatch* h_batch = nullptr;  // pinned
   u32* d_admit = nullptr;
   unsigned long long* d_prepbytes = nullptr;
   u8* d_arena = nullptr;
@@ -304,8 +315,7 @@ struct scs_engine {
   u32* d_trace_cnt = nullptr;
   u32 trace_cap = 0;
   scs::Arena arena;
-  u32 stamp_id = 0;
-  u32* h_cnt = nullptr;  // pinned
+  u32* h_tot = nullptr;  // pinned, 192 words
   hipEvent_t ev0 = nullptr, ev1 = nullptr, pe0 = nullptr, pe1 = nullptr;
   // host
   std::mutex qmu;
@@ -417,25 +427,25 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   if (hipMalloc(&E->d_pool_pos, sizeof(u64) * 2) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_glob, sizeof(EngineGlobals)) != hipSuccess) return fail(SCS_ENOMEM);
   const u32 n1 = std::max<u32>(n, 1);
-  if (hipMalloc(&E->d_stamp, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipMalloc(&E->d_dist, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipMalloc(&E->d_F, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipMalloc(&E->d_Fn, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipMalloc(&E->d_locid, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipMalloc(&E->d_korder, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipMalloc(&E->d_ldeg, sizeof(u32) * n1) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipMalloc(&E->d_bsum, sizeof(u32) * (n1 / 2048 + 64)) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipMalloc(&E->d_cnt, sizeof(u32) * 16) != hipSuccess) return fail(SCS_ENOMEM);
+  const u64 G = (n1 + 63) / 64;
+  for (u64** b : {&E->d_elig, &E->d_vis, &E->d_fr, &E->d_nf})
+    if (hipMalloc(b, sizeof(u64) * n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_dist16, sizeof(u16) * 64 * (u64)n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_locid64, sizeof(u32) * 64 * (u64)n1) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_gcnt, sizeof(u32) * 64 * G) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_flags, sizeof(u32) * SCS_MS_LEVELS) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_tot, sizeof(u32) * 192) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipMalloc(&E->d_batch, sizeof(MsBatch)) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipHostMalloc(&E->h_batch, sizeof(MsBatch)) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_admit, sizeof(u32) * 2 * nq) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_prepbytes, sizeof(unsigned long long)) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_arena, p.mem_budget) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipHostMalloc(&E->h_cnt, sizeof(u32) * 16) != hipSuccess) return fail(SCS_ENOMEM);
+  if (hipHostMalloc(&E->h_tot, sizeof(u32) * 192) != hipSuccess) return fail(SCS_ENOMEM);
   E->arena.init(E->d_arena, p.mem_budget);
   if (hipMemset(E->d_slots, 0, sizeof(QuerySlot) * nq) != hipSuccess) return fail(SCS_EHIP);
   if (hipMemset(E->d_wave, 0, lay.stride * E->nwaves) != hipSuccess) return fail(SCS_EHIP);
   if (hipMemset(E->d_pool_pos, 0, sizeof(u64) * 2) != hipSuccess) return fail(SCS_EHIP);
   if (hipMemset(E->d_glob, 0, sizeof(EngineGlobals)) != hipSuccess) return fail(SCS_EHIP);
-  if (hipMemset(E->d_stamp, 0, sizeof(u32) * n1) != hipSuccess) return fail(SCS_EHIP);
   if (hipMemset(E->d_prepbytes, 0, sizeof(unsigned long long)) != hipSuccess) return fail(SCS_EHIP);
   if (scs_launch_pool_init(E->d_pool_seq, nslots, E->s) != hipSuccess) return fail(SCS_EHIP);
   if (hipStreamSynchronize(E->s) != hipSuccess) return fail(SCS_EHIP);
@@ -529,12 +539,13 @@ extern "C" void scs_engine_destroy(scs_engine* E) {
   if (E->s) (void)hipStreamSynchronize(E->s);
   if (E->ps) (void)hipStreamSynchronize(E->ps);
   void* ptrs[] = {E->d_slots, E->d_witness, E->d_wave, E->d_pool_seq, E->d_pool_data,
-                  E->d_pool_pos, E->d_glob, E->d_stamp, E->d_dist, E->d_F, E->d_Fn, E->d_locid,
-                  E->d_korder, E->d_ldeg, E->d_bsum, E->d_cnt, E->d_admit, E->d_prepbytes,
-                  E->d_arena, E->d_model, E->d_trace, E->d_trace_cnt};
+                  E->d_pool_pos, E->d_glob, E->d_elig, E->d_vis, E->d_fr, E->d_nf, E->d_dist16,
+                  E->d_locid64, E->d_gcnt, E->d_flags, E->d_tot, E->d_batch, E->d_admit,
+                  E->d_prepbytes, E->d_arena, E->d_model, E->d_trace, E->d_trace_cnt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  if (E->h_cnt) (void)hipHostFree(E->h_cnt);
+  if (E->h_tot) (void)hipHostFree(E->h_tot);
+  if (E->h_batch) (void)hipHostFree(E->h_batch);
   for (hipEvent_t ev : {E->ev0, E->ev1, E->pe0, E->pe1})
     if (ev) (void)hipEventDestroy(ev);
   if (E->s) (void)hipStreamDestroy(E->s);
@@ -596,74 +607,9 @@ static void push_result(scs_engine* E, const PendingQuery& pq, u32 kl, u32 heu_k
   E->results.push_back(r);
 }
 
-// GPU preprocessing of one query into slot si.  Returns SCS_OK, SCS_EBIG
-// (kept set above kcap) or SCS_EAGAIN (arena full).
-static int prep_query(scs_engine* E, u32 si, u32 kl0, u32 ubD) {
+// Writes the device record of an admitted query (local graph in the arena).
+static void fill_slot(scs_engine* E, u32 si, const MsQuery& P, u32 qloc) {
   HostSlot& hs = E->slots[si];
-  const scs::HostGraph& hg = E->g->hg;
-  const u32 n = hg.n;
-  hipStream_t s = E->ps;
-  if (++E->stamp_id == 0) {
-    HIPCHK(hipMemsetAsync(E->d_stamp, 0, sizeof(u32) * n, s));
-    E->stamp_id = 1;
-  }
-  const u32 id = E->stamp_id;
-  HIPCHK(hipEventRecord(E->pe0, s));
-  HIPCHK(scs_launch_bfs_seed(hs.pq.q, id, E->d_stamp, E->d_dist, E->d_F, E->d_cnt, s));
-  u32 *F = E->d_F, *Fn = E->d_Fn;
-  u32 *fc = E->d_cnt, *fnc = E->d_cnt + 1;
-  for (u32 level = 0; level < ubD; level++) {
-    HIPCHK(hipMemcpyAsync(E->h_cnt, fc, sizeof(u32), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    const u32 fn = E->h_cnt[0];
-    if (fn == 0) break;
-    HIPCHK(hipMemsetAsync(fnc, 0, sizeof(u32), s));
-    HIPCHK(scs_launch_bfs_level(E->g->d_off, E->g->d_adj, E->g->d_core, kl0, id, level, F, fn,
-                                E->d_stamp, E->d_dist, Fn, fnc, s));
-    std::swap(F, Fn);
-    std::swap(fc, fnc);
-  }
-  HIPCHK(scs_launch_kept_scan(n, E->d_stamp, id, E->d_bsum, E->d_locid, E->d_cnt + 2, s));
-  HIPCHK(hipMemcpyAsync(E->h_cnt + 2, E->d_cnt + 2, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  const u32 K = E->h_cnt[2];
-  hs.K = K;
-  if (K > E->kcap) return SCS_EBIG;
-  HIPCHK(scs_launch_kept_order(n, E->d_stamp, id, E->d_locid, E->d_korder, s));
-  HIPCHK(scs_launch_local_degree(E->g->d_off, E->g->d_adj, E->d_stamp, id, E->d_korder, K,
-                                 E->d_ldeg, s));
-  // block A: loff (K+1) | ldist (K) | lcore (K) | lorig (K).  a_len/b_len
-  // are set only once the allocation succeeded: release_slot frees what
-  // they describe.
-  const u64 a_len = sizeof(u32) * (4ull * K + 1);
-  if (!E->arena.alloc(a_len, &hs.a_off)) return SCS_EAGAIN;
-  hs.a_len = a_len;
-  u32* A = (u32*)(E->d_arena + hs.a_off);
-  u32 *loff = A, *ldist = A + K + 1, *lcore = ldist + K, *lorig = lcore + K;
-  HIPCHK(scs_launch_scan_u32(E->d_ldeg, K, E->d_bsum, loff, E->d_cnt + 3, s));
-  HIPCHK(hipMemcpyAsync(loff + K, E->d_cnt + 3, sizeof(u32), hipMemcpyDeviceToDevice, s));
-  HIPCHK(hipMemcpyAsync(E->h_cnt + 3, E->d_cnt + 3, sizeof(u32), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  const u32 nnzK = E->h_cnt[3];
-  const u64 b_len = sizeof(u32) * (u64)std::max<u32>(nnzK, 1);
-  if (!E->arena.alloc(b_len, &hs.b_off)) {
-    E->arena.release(hs.a_off, hs.a_len);
-    hs.a_len = 0;
-    return SCS_EAGAIN;
-  }
-  hs.b_len = b_len;
-  u32* ladj = (u32*)(E->d_arena + hs.b_off);
-  HIPCHK(scs_launch_local_fill(E->g->d_off, E->g->d_adj, E->d_stamp, id, E->d_locid, E->d_korder,
-                               K, loff, E->d_dist, E->g->d_core, ladj, ldist, lcore, lorig, s));
-  HIPCHK(scs_launch_prep_bytes(E->g->d_off, E->d_korder, K, E->d_prepbytes, s));
-  HIPCHK(hipEventRecord(E->pe1, s));
-  HIPCHK(hipEventSynchronize(E->pe1));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, E->pe0, E->pe1));
-  E->st.prep_us += ms * 1000.0;
-  // slot record
-  u32 qloc = 0;
-  HIPCHK(hipMemcpy(&qloc, E->d_locid + hs.pq.q, sizeof(u32), hipMemcpyDeviceToHost));
   QuerySlot& Q = E->h_slots[si];
   memset(&Q, 0, sizeof(Q));
   Q.state = QS_ACTIVE;
@@ -676,18 +622,150 @@ static int prep_query(scs_engine* E, u32 si, u32 kl0, u32 ubD) {
             (p.learned ? QF_LEARNED : 0);
   Q.tag = (u32)hs.pq.tag;
   Q.ku = hs.ku;
-  Q.K = K;
-  Q.dmax = hg.dmax;
+  Q.K = hs.K;
+  Q.dmax = E->g->hg.dmax;
   Q.qloc = qloc;
-  Q.kl = kl0;
+  Q.kl = hs.kl0;
   Q.pending = 1;
   Q.node_budget = p.node_budget;
-  Q.off = loff;
-  Q.adj = ladj;
-  Q.dist = ldist;
-  Q.orig = lorig;
-  Q.core = lcore;
+  Q.off = P.loff;
+  Q.adj = P.ladj;
+  Q.dist = P.ldist;
+  Q.orig = P.lorig;
+  Q.core = P.lcore;
   Q.witness = E->d_witness + (u64)si * SCS_MAX_H;
+}
+
+// GPU preprocessing of up to 64 queries at once (slots sis[0..cnt), each
+// with its heuristic kl0 and depth bound ubD).  rcs[i] becomes SCS_OK,
+// SCS_EBIG (kept set above kcap) or SCS_EAGAIN (arena full; hs.need holds
+// the bytes known to be needed).  Successful slots get their QuerySlot.
+static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32* ubds, u32 cnt,
+                      int* rcs) {
+  if (cnt == 0 || cnt > 64) return SCS_EINVAL;
+  const scs::HostGraph& hg = E->g->hg;
+  const u32 n = hg.n;
+  const u32 G = (n + 63) / 64;
+  hipStream_t s = E->ps;
+  // bit b = the b-th query by ascending kl0 (elig masks are prefixes)
+  std::vector<u32> ord(cnt);
+  for (u32 i = 0; i < cnt; i++) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](u32 x, u32 y) { return kl0s[x] < kl0s[y]; });
+  MsBatch& B = *E->h_batch;
+  memset(&B, 0, sizeof(B));
+  B.nb = cnt;
+  std::vector<u32> ubd(cnt);
+  u32 maxU = 0;
+  for (u32 b = 0; b < cnt; b++) {
+    const u32 i = ord[b];
+    const HostSlot& hs = E->slots[sis[i]];
+    if (hs.pq.q >= n) return SCS_EINVAL;
+    B.q[b] = hs.pq.q;
+    B.kl0[b] = kl0s[i];
+    ubd[b] = std::min<u32>(ubds[i], SCS_MS_LEVELS);
+    maxU = std::max(maxU, ubd[b]);
+    rcs[i] = SCS_OK;
+  }
+  const u64 all = cnt == 64 ? ~0ull : ((1ull << cnt) - 1ull);
+  const i64* off = E->g->d_off;
+  const int32_t* adj = E->g->d_adj;
+  HIPCHK(hipEventRecord(E->pe0, s));
+  HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
+  HIPCHK(scs_launch_ms_init(n, E->g->d_core, E->d_batch, E->d_elig, E->d_vis, E->d_fr, E->d_flags,
+                            SCS_MS_LEVELS, E->d_prepbytes, s));
+  HIPCHK(scs_launch_ms_seed(E->d_batch, E->d_vis, E->d_fr, E->d_dist16, s));
+  // BFS levels; a level that finds nothing ends the later ones on the
+  // device, and the host checks every 8 levels
+  u64 *fr = E->d_fr, *nf = E->d_nf;
+  u32 L = 0;
+  while (L < maxU) {
+    const u32 stop = std::min(maxU, L + 8);
+    for (; L < stop; L++) {
+      u64 act = 0;
+      for (u32 b = 0; b < cnt; b++)
+        if (ubd[b] > L) act |= 1ull << b;
+      HIPCHK(scs_launch_ms_level(off, adj, n, L, act, E->d_elig, E->d_vis, fr, nf, E->d_dist16,
+                                 E->d_flags + (L ? L - 1 : 0), E->d_flags + L, E->d_prepbytes, s));
+      std::swap(fr, nf);
+    }
+    if (L >= maxU) break;
+    HIPCHK(hipMemcpyAsync(E->h_tot, E->d_flags + (L - 1), sizeof(u32), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (E->h_tot[0] == 0) break;
+  }
+  // kept sets: group counts, per-query bases and sizes
+  HIPCHK(scs_launch_ms_count(n, G, E->d_vis, all, E->d_gcnt, E->d_prepbytes, s));
+  HIPCHK(scs_launch_ms_bscan(cnt, G, E->d_gcnt, E->d_tot, E->d_prepbytes, s));
+  HIPCHK(hipMemcpyAsync(E->h_tot, E->d_tot, sizeof(u32) * 64, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  u64 valid = 0;
+  for (u32 b = 0; b < cnt; b++) {
+    const u32 i = ord[b];
+    HostSlot& hs = E->slots[sis[i]];
+    const u32 K = E->h_tot[b];
+    hs.K = K;
+    if (K > E->kcap || K == 0) {
+      rcs[i] = SCS_EBIG;
+      continue;
+    }
+    // block A: loff (K+1) | ldist (K) | lcore (K) | lorig (K).  a_len/b_len
+    // are set only once the allocation succeeded: release_slot frees what
+    // they describe.
+    const u64 a_len = sizeof(u32) * (4ull * K + 1);
+    if (!E->arena.alloc(a_len, &hs.a_off)) {
+      hs.need = scs::Arena::granule(a_len);
+      rcs[i] = SCS_EAGAIN;
+      continue;
+    }
+    hs.a_len = a_len;
+    u32* A = (u32*)(E->d_arena + hs.a_off);
+    B.P[b].loff = A;
+    B.P[b].ldist = A + K + 1;
+    B.P[b].lcore = A + 2ull * K + 1;
+    B.P[b].lorig = A + 3ull * K + 1;
+    valid |= 1ull << b;
+  }
+  if (valid) {
+    HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
+    HIPCHK(scs_launch_ms_scatter(n, G, E->d_vis, valid, E->d_gcnt, E->g->d_core, E->d_dist16,
+                                 E->d_batch, E->d_locid64, E->d_tot + 128, E->d_prepbytes, s));
+    HIPCHK(scs_launch_ms_degree(off, adj, n, E->d_vis, valid, E->d_locid64, E->d_batch,
+                                E->d_prepbytes, s));
+    HIPCHK(scs_launch_ms_scan_loff(cnt, E->d_batch, E->d_tot, valid, E->d_prepbytes, s));
+    HIPCHK(hipMemcpyAsync(E->h_tot + 64, E->d_tot + 64, sizeof(u32) * 128, hipMemcpyDeviceToHost,
+                          s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (u32 b = 0; b < cnt; b++) {
+      if (!((valid >> b) & 1ull)) continue;
+      const u32 i = ord[b];
+      HostSlot& hs = E->slots[sis[i]];
+      const u32 nnz = E->h_tot[64 + b];
+      const u64 b_len = sizeof(u32) * (u64)std::max<u32>(nnz, 1);
+      if (!E->arena.alloc(b_len, &hs.b_off)) {
+        E->arena.release(hs.a_off, hs.a_len);
+        hs.a_len = 0;
+        hs.need = scs::Arena::granule(sizeof(u32) * (4ull * hs.K + 1)) + scs::Arena::granule(b_len);
+        rcs[i] = SCS_EAGAIN;
+        valid &= ~(1ull << b);
+        continue;
+      }
+      hs.b_len = b_len;
+      B.P[b].ladj = (u32*)(E->d_arena + hs.b_off);
+    }
+    if (valid) {
+      HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
+      HIPCHK(scs_launch_ms_fill(off, adj, n, E->d_vis, valid, E->d_locid64, E->d_batch,
+                                E->d_prepbytes, s));
+    }
+  }
+  HIPCHK(hipEventRecord(E->pe1, s));
+  HIPCHK(hipEventSynchronize(E->pe1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, E->pe0, E->pe1));
+  E->st.prep_us += ms * 1000.0;
+  E->st.prep_batches++;
+  for (u32 b = 0; b < cnt; b++)
+    if ((valid >> b) & 1ull) fill_slot(E, sis[ord[b]], B.P[b], E->h_tot[128 + b]);
   return SCS_OK;
 }
 
@@ -803,8 +881,10 @@ extern "C" int scs_step(scs_engine* E) {
     for (auto& t : th) t.join();
     E->st.heu_us += now_us() - t0;
   }
-  // 3. immediate answers or GPU preprocessing (second stream)
+  // 3. immediate answers; the rest get batched GPU preprocessing (second
+  // stream, 64 queries per pass)
   std::vector<u32> admitted;
+  std::vector<u32> cand_i, cand_si, cand_kl0, cand_ubd;
   size_t fs = 0;
   for (size_t i = 0; i < batch.size(); i++) {
     const PendingQuery& pq = batch[i];
@@ -826,33 +906,45 @@ extern "C" int scs_step(scs_engine* E) {
       push_result(E, pq, kl0[i], kl0[i], ku, SCS_ST_HEU_OPT, 0, 0, H[i].data(), (u32)H[i].size());
       continue;
     }
-    const u32 si = free_slots[fs];
+    const u32 si = free_slots[fs++];
     HostSlot& hs = E->slots[si];
     hs.used = true;
     hs.a_len = hs.b_len = 0;
+    hs.need = 0;
     hs.pq = pq;
     hs.kl0 = kl0[i];
     hs.ku = ku;
     hs.K = 0;
-    int rc = prep_query(E, si, kl0[i], scs::ubd_host(kl0[i], pq.h));
-    if (rc == SCS_EBIG) {
-      push_result(E, pq, kl0[i], kl0[i], ku, SCS_ST_REJECTED, hs.K, 0, H[i].data(), (u32)H[i].size());
-      release_slot(E, si);
-      continue;
-    }
-    if (rc == SCS_EAGAIN) {
-      release_slot(E, si);
-      if (E->arena.empty()) {
-        // the local graph does not fit even in the empty arena
-        push_result(E, pq, kl0[i], kl0[i], ku, SCS_ST_REJECTED, hs.K, 0, H[i].data(),
-                    (u32)H[i].size());
-        continue;
-      }
-      std::lock_guard<std::mutex> lk(E->qmu);
-      E->queue.push_front(pq);
-      continue;
-    }
+    cand_i.push_back((u32)i);
+    cand_si.push_back(si);
+    cand_kl0.push_back(kl0[i]);
+    cand_ubd.push_back(scs::ubd_host(kl0[i], pq.h));
+  }
+  std::vector<int> rcs(cand_i.size(), SCS_OK);
+  for (size_t c0 = 0; c0 < cand_i.size(); c0 += 64) {
+    const u32 cnt = (u32)std::min<size_t>(64, cand_i.size() - c0);
+    int rc = prep_batch(E, &cand_si[c0], &cand_kl0[c0], &cand_ubd[c0], cnt, &rcs[c0]);
     if (rc) return rc;
+  }
+  std::vector<PendingQuery> requeue;
+  for (size_t c = 0; c < cand_i.size(); c++) {
+    const size_t i = cand_i[c];
+    const u32 si = cand_si[c];
+    HostSlot& hs = E->slots[si];
+    const PendingQuery& pq = batch[i];
+    if (rcs[c] == SCS_EBIG || (rcs[c] == SCS_EAGAIN && hs.need > E->arena.size)) {
+      // kept set above the per-query limit, or a local graph larger than
+      // the whole arena
+      push_result(E, pq, kl0[i], kl0[i], hs.ku, SCS_ST_REJECTED, hs.K, 0, H[i].data(),
+                  (u32)H[i].size());
+      release_slot(E, si);
+      continue;
+    }
+    if (rcs[c] == SCS_EAGAIN) {
+      release_slot(E, si);
+      requeue.push_back(pq);
+      continue;
+    }
     QuerySlot& Q = E->h_slots[si];
     if (kl0[i] > 0 && !H[i].empty() && H[i].size() <= SCS_MAX_H) {
       Q.wsize = (u32)H[i].size();
@@ -862,7 +954,10 @@ extern "C" int scs_step(scs_engine* E) {
       hs.seed_witness.clear();
     }
     admitted.push_back(si);
-    fs++;
+  }
+  if (!requeue.empty()) {
+    std::lock_guard<std::mutex> lk(E->qmu);
+    for (auto it = requeue.rbegin(); it != requeue.rend(); ++it) E->queue.push_front(*it);
   }
   // 4. end of the epoch: results of finished queries
   if (running) {
@@ -927,6 +1022,10 @@ extern "C" int scs_engine_stats(const scs_engine* E, scs_stats* s) {
   s->donations = gl.donations;
   s->rebuilds = gl.rebuilds;
   s->edges = gl.edges;
+  for (int i = 0; i < 8; i++) s->phase_ticks[i] = gl.tph[i];
+  s->dom_front = gl.dom_front;
+  s->dom_cand = gl.dom_cand;
+  s->dom_found = gl.dom_found;
   s->core_us = E->g->core_us;
   unsigned long long pb = 0;
   HIPCHK(hipMemcpy(&pb, E->d_prepbytes, sizeof(pb), hipMemcpyDeviceToHost));
@@ -953,36 +1052,63 @@ extern "C" int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_
   return SCS_OK;
 }
 
-extern "C" int scs_prep_probe(scs_engine* E, uint32_t q, uint32_t kl0, uint32_t ubD, uint32_t* K,
-                              uint64_t* nnz, uint32_t* kept, uint32_t* loff, uint32_t* dist,
-                              uint32_t kept_cap, uint32_t* ladj, uint64_t adj_cap) {
-  if (!E || !K || !nnz || q >= E->g->hg.n) return SCS_EINVAL;
-  // use a free slot's bookkeeping temporarily
-  u32 si = 0;
-  while (si < E->p.limit_queries && E->slots[si].used) si++;
-  if (si == E->p.limit_queries) return SCS_EAGAIN;
-  HostSlot& hs = E->slots[si];
-  hs.pq = PendingQuery{0, 1, 1, q, 0, 0, now_us()};
-  hs.ku = 0;
-  int rc = prep_query(E, si, kl0, ubD);
-  *K = hs.K;
-  *nnz = 0;
-  if (rc == SCS_OK) {
-    const QuerySlot& Q = E->h_slots[si];
+extern "C" int scs_prep_probe_many(scs_engine* E, uint32_t count, const uint32_t* q,
+                                   const uint32_t* kl0, const uint32_t* ubD, uint32_t* K,
+                                   uint64_t* nnz, uint32_t* kept, uint32_t* loff, uint32_t* dist,
+                                   uint32_t kept_cap, uint32_t* ladj, uint64_t adj_cap) {
+  if (!E || !q || !kl0 || !ubD || !K || !nnz || count == 0 || count > 64) return SCS_EINVAL;
+  for (u32 i = 0; i < count; i++)
+    if (q[i] >= E->g->hg.n) return SCS_EINVAL;
+  // borrow free slots for the duration of the call
+  std::vector<u32> sis;
+  for (u32 si = 0; si < E->p.limit_queries && sis.size() < count; si++)
+    if (!E->slots[si].used) sis.push_back(si);
+  if (sis.size() < count) return SCS_EAGAIN;
+  for (u32 i = 0; i < count; i++) {
+    HostSlot& hs = E->slots[sis[i]];
+    hs.pq = PendingQuery{0, 1, 1, q[i], 0, 0, now_us()};
+    hs.ku = 0;
+    hs.kl0 = kl0[i];
+    hs.K = 0;
+    hs.a_len = hs.b_len = 0;
+  }
+  std::vector<int> rcs(count, SCS_OK);
+  const int brc = prep_batch(E, sis.data(), kl0, ubD, count, rcs.data());
+  int out = brc;
+  for (u32 i = 0; i < count && brc == SCS_OK; i++) {
+    HostSlot& hs = E->slots[sis[i]];
+    K[i] = hs.K;
+    nnz[i] = 0;
+    if (rcs[i] == SCS_EAGAIN) out = SCS_EAGAIN;
+    if (rcs[i] != SCS_OK) continue;
+    const QuerySlot& Q = E->h_slots[sis[i]];
     const u32 k = hs.K;
     u32 nz = 0;
     HIPCHK(hipMemcpy(&nz, Q.off + k, sizeof(u32), hipMemcpyDeviceToHost));
-    *nnz = nz;
+    nnz[i] = nz;
     if (k <= kept_cap) {
-      if (kept) HIPCHK(hipMemcpy(kept, Q.orig, sizeof(u32) * k, hipMemcpyDeviceToHost));
-      if (loff) HIPCHK(hipMemcpy(loff, Q.off, sizeof(u32) * (k + 1), hipMemcpyDeviceToHost));
-      if (dist) HIPCHK(hipMemcpy(dist, Q.dist, sizeof(u32) * k, hipMemcpyDeviceToHost));
+      if (kept)
+        HIPCHK(hipMemcpy(kept + (u64)i * kept_cap, Q.orig, sizeof(u32) * k, hipMemcpyDeviceToHost));
+      if (loff)
+        HIPCHK(hipMemcpy(loff + (u64)i * (kept_cap + 1ull), Q.off, sizeof(u32) * (k + 1),
+                         hipMemcpyDeviceToHost));
+      if (dist)
+        HIPCHK(hipMemcpy(dist + (u64)i * kept_cap, Q.dist, sizeof(u32) * k, hipMemcpyDeviceToHost));
     }
     if (ladj && nz <= adj_cap)
-      HIPCHK(hipMemcpy(ladj, Q.adj, sizeof(u32) * (u64)nz, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(ladj + (u64)i * adj_cap, Q.adj, sizeof(u32) * (u64)nz,
+                       hipMemcpyDeviceToHost));
   }
-  release_slot(E, si);
-  return rc == SCS_EBIG ? SCS_OK : rc;
+  for (u32 si : sis) release_slot(E, si);
+  return out;
+}
+
+extern "C" int scs_prep_probe(scs_engine* E, uint32_t q, uint32_t kl0, uint32_t ubD, uint32_t* K,
+                              uint64_t* nnz, uint32_t* kept, uint32_t* loff, uint32_t* dist,
+                              uint32_t kept_cap, uint32_t* ladj, uint64_t adj_cap) {
+  if (!E || !K || !nnz) return SCS_EINVAL;
+  return scs_prep_probe_many(E, 1, &q, &kl0, &ubD, K, nnz, kept, loff, dist, kept_cap, ladj,
+                             adj_cap);
 }
 
 extern "C" const char* scs_strerror(int code) {

@@ -55,6 +55,10 @@ __device__ __forceinline__ u64 wmax64(u64 x) {
   }
   return x;
 }
+__device__ __forceinline__ u64 wsum64(u64 x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
 __device__ __forceinline__ u32 bcast(u32 x) { return __shfl(x, 0); }
 __device__ __forceinline__ u64 lanes_below() {
   return LANE == 0 ? 0ull : (~0ull >> (64 - LANE));
@@ -93,6 +97,7 @@ __device__ u32 ubd_node(u32 kl, u32 h) {
 // LDS of one wave (one workgroup).
 struct Lds {
   u32 cnt, cnt2, nfront, ncand, nh, nk, pad0, pad1;
+  u64 tph[SCS_PHASES];      // wall-clock ticks per phase (lane 0 adds)
   u32 D[SCS_FRAME_DOMS];
   u32 hist[SCS_HCAP + 4];   // dC histogram of R (live copy of the wave's state)
   union {                   // never live at the same time within a node
@@ -103,6 +108,23 @@ struct Lds {
 
 // ---------------------------------------------------------------------------
 // Per-wave state view.
+// Phases of the search whose wall-clock time is accounted separately.
+#define PH_REBUILD 0  // state rebuilt from a pooled task
+#define PH_RULES 1    // rule 3 and rule 1/2 closure, kl offers
+#define PH_BOUNDS 2   // U_d and U_nr
+#define PH_SCORE 3    // connection scores, argmax, learned order
+#define PH_DOMS 4     // dominated set of ustar
+#define PH_MOVE 5     // applying children, backtracking (undo)
+#define PH_DONATE 6   // serialising a donated task
+#define PH_IDLE 7     // waiting for a task
+
+__device__ __forceinline__ void ph_switch(Lds* L, u64& tp, u32& ph, u32 next) {
+  const u64 t = wall_clock64();
+  if ((threadIdx.x & 63) == 0) L->tph[ph] += t - tp;
+  tp = t;
+  ph = next;
+}
+
 // ---------------------------------------------------------------------------
 struct W {
   u8* st;
@@ -136,6 +158,7 @@ struct W {
   u32* trace;
   u32* trace_cnt;
   u32 trace_cap, trace_stride;
+  EngineGlobals* gl;
   // wave-uniform counters
   u32 nC, nR, ntrail, nframes, nsav;
   u64 edges;
@@ -165,6 +188,7 @@ __device__ void bind(W& w, const EngineArgs& A, u32 wave, Lds* L) {
   w.edges = 0;
   w.nsav = 0;
   w.model = A.model;
+  w.gl = A.glob;
   w.trace = A.trace;
   w.trace_cnt = A.trace_cnt;
   w.trace_cap = A.trace_cap;
@@ -185,6 +209,58 @@ __device__ void bind_query(W& w, QuerySlot* Q) {
   w.limit_doms = Q->limit_doms;
   w.dmax = Q->dmax;
   w.ku = Q->ku;
+}
+
+// ---------------------------------------------------------------------------
+// Load-balanced adjacency scans over a list of vertices.  vertex(i) gives
+// the i-th vertex (i < m).  Vertices of degree <= SEQ_DEG are taken one per
+// lane; larger ones (power-law hubs) by the whole wave, 64 entries at a
+// time, so no lane walks a long list alone.  edge(x) returns a
+// contribution (AccT) of adjacency entry x; done(i, v, total) runs once
+// per vertex with the sum of its contributions, on one lane.
+// ---------------------------------------------------------------------------
+#define SEQ_DEG 32u
+
+template <typename AccT>
+__device__ __forceinline__ AccT wsum_t(AccT x);
+template <>
+__device__ __forceinline__ u32 wsum_t<u32>(u32 x) { return wsum(x); }
+template <>
+__device__ __forceinline__ u64 wsum_t<u64>(u64 x) { return wsum64(x); }
+
+template <typename AccT, typename VertexF, typename EdgeF, typename DoneF>
+__device__ void scan_adj(W& w, u32 m, VertexF vertex, EdgeF edge, DoneF done) {
+  u32 ed = 0;
+  for (u32 c0 = 0; c0 < m; c0 += 64) {
+    const u32 i = c0 + LANE;
+    u32 v = 0, b = 0, e = 0;
+    if (i < m) {
+      v = vertex(i);
+      b = w.off[v];
+      e = w.off[v + 1];
+    }
+    const bool big = i < m && (e - b) > SEQ_DEG;
+    if (i < m && !big) {
+      AccT acc = 0;
+      for (u32 j = b; j < e; j++) acc += edge(w.adj[j]);
+      done(i, v, acc);
+      ed += e - b;
+    }
+    u64 bm = __ballot(big);
+    while (bm) {
+      const int src = __ffsll((unsigned long long)bm) - 1;
+      bm &= bm - 1;
+      const u32 bv = __shfl(v, src), bb = __shfl(b, src), be = __shfl(e, src);
+      AccT acc = 0;
+      for (u32 j = bb + LANE; j < be; j += 64) acc += edge(w.adj[j]);
+      acc = wsum_t<AccT>(acc);
+      if (LANE == 0) {
+        done(c0 + (u32)src, bv, acc);
+        ed += be - bb;
+      }
+    }
+  }
+  w.edges += wsum(ed);
 }
 
 // ---------------------------------------------------------------------------
@@ -296,14 +372,13 @@ __device__ void remove_batch(W& w, const u32* list, u32 m, const Clo& c) {
   // phase 1 (exact mode): decrement the live neighbours (batch included)
   if (!recount) {
     if (m >= 8) {
-      u32 ed = 0;
-      for (u32 i = LANE; i < m; i += 64) {
-        u32 v = list[i];
-        const u32 b = w.off[v], e = w.off[v + 1];
-        ed += e - b;
-        for (u32 j = b; j < e; j++) dec_neighbour(w, w.adj[j], c);
-      }
-      w.edges += wsum(ed);
+      scan_adj<u32>(
+          w, m, [&](u32 i) { return list[i]; },
+          [&](u32 x) {
+            dec_neighbour(w, x, c);
+            return 0u;
+          },
+          [&](u32, u32, u32) {});
     } else {
       for (u32 i = 0; i < m; i++) {
         u32 v = list[i];
@@ -352,24 +427,21 @@ __device__ void remove_batch(W& w, const u32* list, u32 m, const Clo& c) {
   if (recount) {
     // phase 4: save and recount dR of the live vertices (R list, then C)
     u32* S = w.sav + w.nsav;
-    u32 ed = 0;
-    for (u32 i = LANE; i < live_after; i += 64) {
-      const u32 v = i < w.nR ? w.rlist[i] : w.clist[i - w.nR];
-      const u32 b = w.off[v], e = w.off[v + 1];
-      u32 r = 0;
-      for (u32 j = b; j < e; j++) r += (w.st[w.adj[j]] == ST_R);
-      ed += e - b;
-      S[2 * i] = v;
-      S[2 * i + 1] = w.dR[v];
-      w.dR[v] = r;
-      if (c.on && i < w.nR && w.dC[v] + r <= c.kl) {
-        u32* wp = (u32*)((uintptr_t)(w.queued + v) & ~(uintptr_t)3);
-        u32 sh = ((uintptr_t)(w.queued + v) & 3) * 8;
-        u32 o2 = atomicOr(wp, 1u << sh);
-        if (!((o2 >> sh) & 0xffu)) c.next[atomicAdd(c.cnt, 1u)] = v;
-      }
-    }
-    w.edges += wsum(ed);
+    const u32 nR = w.nR;
+    scan_adj<u32>(
+        w, live_after, [&](u32 i) { return i < nR ? w.rlist[i] : w.clist[i - nR]; },
+        [&](u32 x) { return (u32)(w.st[x] == ST_R); },
+        [&](u32 i, u32 v, u32 r) {
+          S[2 * i] = v;
+          S[2 * i + 1] = w.dR[v];
+          w.dR[v] = r;
+          if (c.on && i < nR && w.dC[v] + r <= c.kl) {
+            u32* wp = (u32*)((uintptr_t)(w.queued + v) & ~(uintptr_t)3);
+            u32 sh = ((uintptr_t)(w.queued + v) & 3) * 8;
+            u32 o2 = atomicOr(wp, 1u << sh);
+            if (!((o2 >> sh) & 0xffu)) c.next[atomicAdd(c.cnt, 1u)] = v;
+          }
+        });
     if (LANE == 0) S[2 * live_after] = live_after;
     w.nsav += 2 * live_after + 1;
     wsync();
@@ -398,17 +470,13 @@ __device__ void undo_batch(W& w, u32 start, u32 m, bool recount) {
   wsync();
   if (recount) return;
   if (m >= 8) {
-    u32 ed = 0;
-    for (u32 i = LANE; i < m; i += 64) {
-      u32 v = E[i];
-      const u32 b = w.off[v], e = w.off[v + 1];
-      ed += e - b;
-      for (u32 j = b; j < e; j++) {
-        u32 x = w.adj[j];
-        if (w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
-      }
-    }
-    w.edges += wsum(ed);
+    scan_adj<u32>(
+        w, m, [&](u32 i) { return E[i]; },
+        [&](u32 x) {
+          if (w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
+          return 0u;
+        },
+        [&](u32, u32, u32) {});
   } else {
     for (u32 i = 0; i < m; i++) {
       u32 v = E[i];
@@ -634,6 +702,7 @@ __device__ __forceinline__ MlpFeat cand_features(const W& w, u32 x, u32 Cs, u32 
 __device__ u32 learned_ustar(W& w, u32 nf, u32 Cs, u32 kl) {
   float bestf = -INFINITY;
   u32 bestx = INV;
+#pragma unroll 1
   for (u32 t0 = 0; t0 < nf; t0 += 16) {
     auto feat = [&](u32 row) {
       const u32 i = t0 + row;
@@ -676,7 +745,10 @@ __device__ void trace_node(W& w, u32 nf, u32 Cs, u32 kl) {
   }
 }
 
-__device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno) {
+// LEARN selects the kernel variant that can score candidates with the MLP;
+// the plain variant leaves that path out and needs half the registers.
+template <bool LEARN>
+__device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32& ph) {
   Lds* L = w.L;
   NodeOut o;
   o.branch = 0;
@@ -741,6 +813,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno) {
   const u32 Cs = w.nC;
   if (Cs + w.nR < w.l || w.nR == 0) return o;
   // bounds
+  ph_switch(L, tp, ph, PH_BOUNDS);
   u32 ub = INV;
   if (w.flags & QF_PRUN1) {
     u32 b = INV;
@@ -754,6 +827,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno) {
   if ((w.flags & QF_PRUN2) && ub > kl) ub = min(ub, unr_bound(w, min(h - Cs, w.nR)));
   if (ub <= kl) return o;
   // connection scores by scattering from C into acc[]; frontier = R with dC>0
+  ph_switch(L, tp, ph, PH_SCORE);
   if (LANE == 0) L->nfront = 0;
   wsync();
   const bool single0 = (Cs == 1 && w.dC[w.clist[0]] == 0);
@@ -785,7 +859,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno) {
   const u64 gbest = wmax64(best);
   u32 us = wmin(best == gbest ? bid : INV);
   if (nf > 0 && w.trace && (nodeno % w.trace_stride) == 0) trace_node(w, nf, Cs, kl);
-  if (nf > 0 && us != INV && (w.flags & QF_LEARNED) && w.model) {
+  if (LEARN && nf > 0 && us != INV && (w.flags & QF_LEARNED) && w.model) {
     // learned order: any frontier vertex is a valid branching vertex
     const u32 ul = learned_ustar(w, nf, Cs, kl);
     if (ul != INV) us = ul;
@@ -796,6 +870,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno) {
     return o;
   }
   // dominated set of us
+  ph_switch(L, tp, ph, PH_DOMS);
   u32 nD = 0;
   if ((w.flags & QF_DOMS) && w.limit_doms > 0) {
     const u32 lu = w.dC[us] + w.dR[us];
@@ -803,10 +878,13 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno) {
     for (u32 j = ub0 + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 1;
     if (LANE == 0) L->ncand = 0;
     wsync();
-    u32 ed = 0;
+    // x is dominated when every live neighbour is us or a neighbour of us;
+    // one candidate per lane: the scans mostly stop after a few entries
+    u32 ed = 0, ntest = 0;
     for (u32 i = LANE; i < nf; i += 64) {
       u32 x = w.front[i];
       if (x == us || w.dC[x] + w.dR[x] > lu + 1) continue;
+      ntest++;
       bool dom = true;
       const u32 b = w.off[x], e = w.off[x + 1];
       for (u32 j = b; j < e; j++) {
@@ -824,6 +902,12 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno) {
     wsync();
     for (u32 j = ub0 + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 0;
     const u32 nc = bcast(L->ncand);
+    ntest = wsum(ntest);
+    if (LANE == 0) {
+      atomicAdd((unsigned long long*)&w.gl->dom_front, (unsigned long long)nf);
+      atomicAdd((unsigned long long*)&w.gl->dom_cand, (unsigned long long)ntest);
+      atomicAdd((unsigned long long*)&w.gl->dom_found, (unsigned long long)nc);
+    }
     // top limit_doms by (score desc, id asc); the score counts C neighbours
     // with dC != 0 (only C = {q} with dC(q) = 0 differs from acc)
     const u32 want = min(nc, w.limit_doms);
@@ -938,23 +1022,20 @@ __device__ void rebuild(W& w, const u32* task) {
   w.nR = base;
   w.nC = nC;
   wsync();
-  // degrees of live vertices, one lane per vertex
-  u32 ed = 0;
-  for (u32 i = LANE; i < w.nR + nC; i += 64) {
-    u32 v = i < w.nR ? w.rlist[i] : w.clist[i - w.nR];
-    u32 c = 0, r = 0;
-    const u32 b = w.off[v], e = w.off[v + 1];
-    for (u32 j = b; j < e; j++) {
-      u8 s = w.st[w.adj[j]];
-      c += (s == ST_C);
-      r += (s == ST_R);
-    }
-    ed += e - b;
-    w.dC[v] = (u16)c;
-    w.dR[v] = r;
-    if (i < w.nR) atomicAdd(&w.L->hist[min(c, w.h + 1)], 1u);
-  }
-  w.edges += wsum(ed);
+  // degrees of live vertices (dC in the high word, dR in the low word)
+  const u32 nR = w.nR;
+  scan_adj<u64>(
+      w, nR + nC, [&](u32 i) { return i < nR ? w.rlist[i] : w.clist[i - nR]; },
+      [&](u32 x) {
+        const u8 s = w.st[x];
+        return ((u64)(s == ST_C) << 32) | (u64)(s == ST_R);
+      },
+      [&](u32 i, u32 v, u64 cr) {
+        const u32 c = (u32)(cr >> 32), r = (u32)cr;
+        w.dC[v] = (u16)c;
+        w.dR[v] = r;
+        if (i < nR) atomicAdd(&w.L->hist[min(c, w.h + 1)], 1u);
+      });
   w.ntrail = 0;
   w.nframes = 0;
   w.nsav = 0;
@@ -1058,7 +1139,8 @@ __device__ void flush_counters(W& w, const EngineArgs& A, u32& pend) {
 // ---------------------------------------------------------------------------
 // The persistent epoch kernel.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
+template <bool LEARN>
+__global__ void __launch_bounds__(64, 4) scs_epoch(EngineArgs A) {
   __shared__ Lds L;
   const u32 wave = blockIdx.x;
   if (wave >= A.nwaves) return;
@@ -1066,6 +1148,10 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
   const u64 deadline = t0 + A.slice_ticks;
   W w;
   bind(w, A, wave, &L);
+  if (LANE < SCS_PHASES) L.tph[LANE] = 0;
+  wsync();
+  u64 tp = t0;
+  u32 ph = PH_IDLE;
   WaveHdr* H = w.hdr;
   u32 active = H->active, qslot = H->qslot;
   u32 from = H->from, kl_seen = H->kl_seen, pend = 0;
@@ -1085,6 +1171,7 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
     if (wall_clock64() > deadline) break;
     if (!active) {
       // take a task from the pool
+      if (ph != PH_IDLE) ph_switch(&L, tp, ph, PH_IDLE);
       u64 pos = 0;
       u32 got = 0;
       if (LANE == 0) got = pool_try_dequeue(A.pool, &pos);
@@ -1096,13 +1183,19 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
         }
         if (aload(&A.glob->active_q) == 0) break;
         if (++idle_spins > (1u << 22)) break;
-        __builtin_amdgcn_s_sleep(20);
+        // back off: idle waves polling the pool contend with busy waves'
+        // atomics on the same cache lines
+        if (idle_spins < 8)
+          __builtin_amdgcn_s_sleep(8);
+        else
+          __builtin_amdgcn_s_sleep(127);
         continue;
       }
       if (idle_counted) {
         if (LANE == 0) atomicSub(&A.glob->idle, 1u);
         idle_counted = false;
       }
+      idle_spins = 0;
       u32 plo = __shfl((u32)pos, 0), phi = __shfl((u32)(pos >> 32), 0);
       pos = ((u64)phi << 32) | plo;
       const u32* task = A.pool.data + (pos % A.pool.nslots) * (u64)A.pool.slot_words;
@@ -1115,6 +1208,7 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
         finish_task(w, A, qslot);
         continue;
       }
+      ph_switch(&L, tp, ph, PH_REBUILD);
       rebuild(w, task);
       __threadfence();
       if (LANE == 0) {
@@ -1149,7 +1243,9 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
       finish_task(w, A, qslot);
       continue;
     }
-    NodeOut o = process(w, from, kl_seen, w.trace ? aload64(&Q->nodes) + pend : 0ull);
+    ph_switch(&L, tp, ph, PH_RULES);
+    NodeOut o = process<LEARN>(w, from, kl_seen, w.trace ? aload64(&Q->nodes) + pend : 0ull, tp, ph);
+    ph_switch(&L, tp, ph, PH_MOVE);
     pend++;
     if (pend >= A.node_flush) flush_counters(w, A, pend);
     if (o.branch) {
@@ -1245,7 +1341,9 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
             u32* dst = A.pool.data + (pos % A.pool.nslots) * (u64)A.pool.slot_words;
             if (LANE == 0) dst[3] = form;
             wsync();
+            ph_switch(&L, tp, ph, PH_DONATE);
             serialize_z(w, fi, dst, qslot);
+            ph_switch(&L, tp, ph, PH_MOVE);
             if (LANE == 0) {
               w.frames[fi].zgone = 1;
               atomicAdd(&Q->pending, 1u);
@@ -1262,6 +1360,10 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
     }
   }
   if (idle_counted && LANE == 0) atomicSub(&A.glob->idle, 1u);
+  ph_switch(&L, tp, ph, ph);
+  wsync();
+  if (LANE < SCS_PHASES && L.tph[LANE])
+    atomicAdd((unsigned long long*)&A.glob->tph[LANE], (unsigned long long)L.tph[LANE]);
   if (active) {
     flush_counters(w, A, pend);
     save_state(w, 1, qslot, from, kl_seen);
@@ -1271,7 +1373,10 @@ __global__ void __launch_bounds__(64) scs_epoch(EngineArgs A) {
 }
 
 extern "C" hipError_t scs_launch_epoch(const EngineArgs* A, hipStream_t s) {
-  scs_epoch<<<A->nwaves, 64, 0, s>>>(*A);
+  if (A->model)
+    scs_epoch<true><<<A->nwaves, 64, 0, s>>>(*A);
+  else
+    scs_epoch<false><<<A->nwaves, 64, 0, s>>>(*A);
   return hipGetLastError();
 }
 

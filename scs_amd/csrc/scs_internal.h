@@ -14,6 +14,7 @@ typedef int64_t i64;
 #define SCS_WAVE 64
 #define SCS_FRAME_DOMS SCS_MAX_DOMS
 #define SCS_HCAP (SCS_MAX_H + 2)
+#define SCS_PHASES 8
 
 // vertex status inside a search state
 #define ST_R 0
@@ -116,9 +117,31 @@ struct EngineGlobals {
   u64 edges;       // adjacency entries scanned by search waves
   u32 lock_fail;
   u32 pad[5];
+  u64 dom_front;  // frontier vertices over all nodes that branched
+  u64 dom_cand;   // of them, candidates whose adjacency was tested
+  u64 dom_found;  // of them, dominated by ustar
+  u64 tph[SCS_PHASES];  // wall-clock ticks (100 MHz) per search phase, all waves
 };
 
 struct MlpDev;
+
+// Batched preprocessing: device pointers of one query's local graph, and
+// the batch header (query b of the batch owns bit b; queries ordered by
+// ascending kl0).
+struct MsQuery {
+  u32* loff;   // K+1
+  u32* ldist;  // K
+  u32* lcore;  // K
+  u32* lorig;  // K
+  u32* ladj;   // nnz
+};
+struct MsBatch {
+  u32 nb;
+  u32 pad;
+  u32 q[64];
+  u32 kl0[64];
+  MsQuery P[64];
+};
 
 struct EngineArgs {
   const MlpDev* model;  // branching MLP (queries with QF_LEARNED)

@@ -74,6 +74,7 @@ __device__ float mlp_tile(const MlpDev* M, FeatFn feat, u16* h1s) {
     if (k < MLP_IN) x = (xin[k] - M->mean[k]) * M->scale[k];
     a1[i] = (short)f2bf(x);
   }
+#pragma unroll 1
   for (u32 j = 0; j < 4; j++) {
     bf16x4 b;
     for (int i = 0; i < 4; i++) b[i] = (short)M->w1[4 * kb + i][16 * j + row];
@@ -89,8 +90,10 @@ __device__ float mlp_tile(const MlpDev* M, FeatFn feat, u16* h1s) {
   __syncthreads();
   // layer 2 (K = 64 in four steps) fused with the 64 -> 1 output layer
   float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
   for (u32 j = 0; j < 4; j++) {
     f32x4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
     for (u32 ks = 0; ks < 4; ks++) {
       bf16x4 a, b;
       for (int i = 0; i < 4; i++) {

@@ -60,7 +60,9 @@ class Stats(C.Structure):
     _fields_ = [("nodes", C.c_uint64), ("donations", C.c_uint64), ("rebuilds", C.c_uint64),
                 ("engine_us", C.c_double), ("prep_us", C.c_double), ("heu_us", C.c_double),
                 ("prep_bytes", C.c_uint64), ("epochs", C.c_uint64), ("waves", C.c_uint32),
-                ("kcap", C.c_uint32), ("edges", C.c_uint64), ("core_us", C.c_double)]
+                ("kcap", C.c_uint32), ("edges", C.c_uint64), ("core_us", C.c_double),
+                ("prep_batches", C.c_uint64), ("phase_ticks", C.c_uint64 * 8),
+                ("dom_front", C.c_uint64), ("dom_cand", C.c_uint64), ("dom_found", C.c_uint64)]
 
 
 class Model(C.Structure):
@@ -328,7 +330,9 @@ class Engine:
     def stats(self):
         s = Stats()
         _check(lib().scs_engine_stats(self._h, C.byref(s)), "stats")
-        return {f: getattr(s, f) for f, _ in Stats._fields_}
+        out = {f: getattr(s, f) for f, _ in Stats._fields_}
+        out["phase_ticks"] = list(out["phase_ticks"])
+        return out
 
     def close(self):
         if self._h:

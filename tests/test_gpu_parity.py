@@ -88,6 +88,48 @@ def test_preprocessing_matches_oracle(graphs, name):
     E.close()
 
 
+@pytest.mark.parametrize("name", ["hepPH_SCS", "C1"])
+def test_batched_preprocessing_matches_oracle(graphs, name):
+    """64 queries with mixed kl0 (including kl0 <= 1, whose depth bound is
+    h - 1) in one batched pass: every query's local graph equals the
+    oracle's preprocessing of that query alone."""
+    import ctypes as C
+    G, og = graphs[name]
+    E = scs.Engine(G, limit_queries=64, waves_per_cu=1)
+    core = og.core()
+    rng = np.random.default_rng(23)
+    n = og.n
+    cnt = 64
+    qs = rng.integers(0, n, cnt).astype(np.uint32)
+    kl0 = np.array([int(rng.integers(0, max(1, int(core[q])))) if i % 4 else 0 for i, q in enumerate(qs)],
+                   np.uint32)
+    hs = rng.integers(3, 40, cnt)
+    ubd = np.array([ubd_host(int(k), int(h)) for k, h in zip(kl0, hs)], np.uint32)
+    adjcap = int(og.csr()[0][-1]) + 1
+    kept = np.zeros(cnt * n, np.uint32)
+    loff = np.zeros(cnt * (n + 1), np.uint32)
+    dist = np.zeros(cnt * n, np.uint32)
+    ladj = np.zeros(cnt * adjcap, np.uint32)
+    K = np.zeros(cnt, np.uint32)
+    nnz = np.zeros(cnt, np.uint64)
+    f = scs.lib().scs_prep_probe_many
+    u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+    f.argtypes = [C.c_void_p, C.c_uint32, u32p, u32p, u32p, u32p,
+                  np.ctypeslib.ndpointer(np.uint64, flags="C"), u32p, u32p, u32p, C.c_uint32, u32p,
+                  C.c_uint64]
+    print(name, "batched probe of", cnt, "queries", flush=True)
+    assert f(E._h, cnt, qs, kl0, ubd, K, nnz, kept, loff, dist, n, ladj, adjcap) == 0
+    for i in range(cnt):
+        ref = og.preprocess(int(qs[i]), int(kl0[i]), int(ubd[i]))
+        k = int(K[i])
+        assert k == ref["K"], i
+        assert np.array_equal(kept[i * n:i * n + k], ref["kept"]), i
+        assert np.array_equal(loff[i * (n + 1):i * (n + 1) + k + 1], ref["off"]), i
+        assert np.array_equal(dist[i * n:i * n + k], ref["dist"]), i
+        assert np.array_equal(ladj[i * adjcap:i * adjcap + int(nnz[i])], ref["adj"]), i
+    E.close()
+
+
 def run_engine(G, queries, **kw):
     E = scs.Engine(G, **kw)
     res = E.run(queries)
