@@ -57,15 +57,38 @@ def main():
     ap.add_argument("--queries", type=int, default=2000)
     ap.add_argument("--budget", type=int, default=300, help="node budget per query")
     # defaults: the autotuner's pick for this workload (scs_amd/tuned_c2.json)
-    ap.add_argument("--waves-per-cu", type=int, default=4)
-    ap.add_argument("--limit-queries", type=int, default=512)
+    ap.add_argument("--heuristic-threads", type=int, default=12)
+    ap.add_argument("--waves-per-cu", type=int, default=8)
+    ap.add_argument("--limit-queries", type=int, default=2048)
     ap.add_argument("--slice-us", type=int, default=5000)
     ap.add_argument("--copy-limit", type=float, default=0.5)
-    ap.add_argument("--learned", type=int, default=0)
+    ap.add_argument("--learned", type=int, default=0,
+                    help="1 = the main leg itself uses the learned order")
+    ap.add_argument("--model", default=os.path.join(ROOT, "models", "branching_c1.npz"),
+                    help="branching model for the learned leg (skipped if the file is absent)")
+    ap.add_argument("--prep-sample", type=int, default=512,
+                    help="queries preprocessed alone (batched) for the preprocessing roofline")
+    ap.add_argument("--single-sample", type=int, default=24,
+                    help="queries run one at a time (plain order) for the one-query GPU form")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
+    ap.add_argument("--copy-calibrate-mib", type=int, default=0,
+                    help="only run a plain elementwise pass over this many MiB (read once, "
+                         "written once) for scaling hardware traffic counters, then exit")
     args = ap.parse_args()
+
+    if args.copy_calibrate_mib:
+        import torch
+        n = args.copy_calibrate_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "elementwise mul", "bytes_read": n * 4,
+                          "bytes_written": n * 4, "dispatches": 3}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); queries are independent "
@@ -74,14 +97,23 @@ def main():
 
     from scs_amd import configs, graphgen as gg, scs
 
+    print(f"[bench] building {args.config}", file=sys.stderr, flush=True)
     g = configs.graph(args.config)
+    print(f"[bench] graph n={g.n} m={g.m}", file=sys.stderr, flush=True)
     G = scs.Graph.from_csr(g.off, g.adj)
     core = G.core()
     qs = configs.queries(args.config, core.astype(np.int64), count=args.queries)
     queries = [(q.l, q.h, q.q, q.heu, q.limit_doms) for q in qs]
-    E = scs.Engine(G, node_budget=args.budget, waves_per_cu=args.waves_per_cu,
-                   limit_queries=args.limit_queries, time_slice_us=args.slice_us,
-                   copy_limit=args.copy_limit, learned=args.learned, mem_budget=32 << 30)
+    model = scs.Model.load(args.model) if args.model and os.path.exists(args.model) else None
+    eng_kw = dict(node_budget=args.budget, waves_per_cu=args.waves_per_cu,
+                  limit_queries=args.limit_queries, time_slice_us=args.slice_us,
+                  copy_limit=args.copy_limit, mem_budget=32 << 30,
+                  heuristic_threads=args.heuristic_threads)
+    E = scs.Engine(G, **eng_kw)
+    if args.learned:
+        if model is None:
+            sys.exit("--learned needs --model")
+        E.set_model(model, True)
     # load code objects outside the timed region
     E.run([queries[0]])
 
@@ -146,6 +178,77 @@ def main():
         "heuristic_host_us": round(s1["heu_us"] - s0["heu_us"], 1),
         "core_decomposition_us": round(s1["core_us"], 1),
     }
+    line["kl_sum"] = int(sum(r.kl for r in all_res))
+    line["kcap"] = int(s1["kcap"])
+    E.close()
+
+    # the same queries in the learned order (batched), and the plain order
+    # one query at a time
+    if model is not None and not args.learned:
+        print("[bench] learned leg", file=sys.stderr, flush=True)
+        EL = scs.Engine(G, **eng_kw)
+        EL.set_model(model, True)
+        EL.run([queries[0]])
+        t0 = time.perf_counter()
+        lres = EL.run(queries, progress=10.0)
+        lw = time.perf_counter() - t0
+        EL.close()
+        same = all(a.kl == b.kl for a, b in zip(all_res, lres)
+                   if a.status != "CAPPED" and b.status != "CAPPED")
+        llat = [r.latency_us / 1000.0 for r in lres]
+        line["learned_batched"] = {
+            "value": round(len(queries) / lw, 3), "unit": "queries/s",
+            "p50_ms": round(percentile(llat, 50), 3), "p99_ms": round(percentile(llat, 99), 3),
+            "kl_sum": int(sum(r.kl for r in lres)),
+            "proven": int(sum(r.status in ("OK", "HEU_OPT", "INFEASIBLE") for r in lres)),
+            "optimum_equal_where_both_complete": same, "model": os.path.basename(args.model)}
+    if args.single_sample > 0:
+        print("[bench] one-query-at-a-time leg", file=sys.stderr, flush=True)
+        sq = queries[:args.single_sample]
+        ES = scs.Engine(G, **dict(eng_kw, limit_queries=1))
+        ES.run([queries[0]])
+        t0 = time.perf_counter()
+        sres = ES.run(sq, progress=10.0)
+        sw = time.perf_counter() - t0
+        ES.close()
+        slat = [r.latency_us / 1000.0 for r in sres]
+        line["plain_one_at_a_time"] = {
+            "value": round(len(sq) / sw, 3), "unit": "queries/s", "sample": len(sq),
+            "p99_ms": round(percentile(slat, 99), 3)}
+        line["batched_vs_one_at_a_time"] = round(qps / (len(sq) / sw), 2)
+        if "learned_batched" in line:
+            line["learned_batched_vs_one_at_a_time"] = round(
+                line["learned_batched"]["value"] / (len(sq) / sw), 2)
+
+    # batched preprocessing alone (no search kernel sharing the GPU): the
+    # roofline of the preprocessing kernels
+    if args.prep_sample > 0:
+        print("[bench] standalone preprocessing leg", file=sys.stderr, flush=True)
+        EP = scs.Engine(G, limit_queries=64, waves_per_cu=1, mem_budget=32 << 30)
+        core64 = core.astype(np.int64)
+        todo = []
+        for (l, h, q, heu, ld) in queries[:args.prep_sample]:
+            kl0, ubd = EP.heuristic(l, h, q)
+            if kl0 < min(int(core64[q]), h - 1):
+                todo.append((q, kl0, ubd))
+        EP.prep_many(*zip(*todo[:1])) if todo else None  # code objects
+        p0 = EP.stats()
+        Ks = []
+        for c0 in range(0, len(todo), 64):
+            qq, kk, uu = zip(*todo[c0:c0 + 64])
+            K, _ = EP.prep_many(qq, kk, uu)
+            Ks.extend(int(x) for x in K)
+        p1 = EP.stats()
+        EP.close()
+        pus = p1["prep_us"] - p0["prep_us"]
+        pby = p1["prep_bytes"] - p0["prep_bytes"]
+        gbs = pby / (pus * 1e-6) / 1e9 if pus > 0 else 0.0
+        line["prep_standalone"] = {
+            "queries": len(todo), "batches": int(p1["prep_batches"] - p0["prep_batches"]),
+            "device_us": round(pus, 1), "us_per_query": round(pus / max(1, len(todo)), 2),
+            "algorithmic_bytes": int(pby), "achieved_GBs": round(gbs, 2),
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "mean_kept": round(float(np.mean(Ks)), 1) if Ks else 0}
+
     pb = line["prep"]["algorithmic_bytes"]
     pu = line["prep"]["device_us"]
     line["prep"]["achieved_GBs"] = round(pb / (pu * 1e-6) / 1e9, 2) if pu > 0 else 0.0
@@ -168,7 +271,6 @@ def main():
                                 "kind": "port",
                                 "sample": f"first {done} queries of the same stream, node budget "
                                           f"{args.budget}, serial trail solver, {tc:.1f} s"}
-    E.close()
     G.close()
     out = json.dumps(line)
     print(out, flush=True)

@@ -91,8 +91,10 @@ typedef struct scs_params {
                               counts equal the CPU oracle's */
   uint32_t learned;        /* 1 = MLP scores the branching vertex */
   const void* model;       /* const scs_model* used when learned */
-  uint32_t heuristic_threads; /* host threads for CSSC_heu (0 = 4) */
-  uint32_t reserved[7];
+  uint32_t heuristic_threads; /* host threads for CSSC_heu (0 = 8) */
+  uint32_t admit_per_step; /* queries admitted per scs_step (0 = 256): the
+                              search starts while later arrivals are prepared */
+  uint32_t reserved[6];
 } scs_params;
 
 typedef struct scs_result {
@@ -183,6 +185,11 @@ int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_t n, uint32
 int scs_prep_probe(scs_engine* e, uint32_t q, uint32_t kl0, uint32_t ubD, uint32_t* K,
                    uint64_t* nnz, uint32_t* kept, uint32_t* loff, uint32_t* dist,
                    uint32_t kept_cap, uint32_t* ladj, uint64_t adj_cap);
+/* The engine's own admission bound for one query: the CSSC_heu lower bound
+ * kl (inc/heuristic.h:321-348) on the engine's graph and the diameter bound
+ * ubD(kl, h) of main.cu:154-172 that limits the BFS. */
+int scs_engine_heuristic(scs_engine* e, uint32_t l, uint32_t h, uint32_t q, uint32_t* kl,
+                         uint32_t* ubD);
 /* The same for count <= 64 queries preprocessed together in one batched
  * pass.  Query i writes K[i], nnz[i], and (when they fit) kept/dist at
  * i*kept_cap, loff at i*(kept_cap+1) and ladj at i*adj_cap. */

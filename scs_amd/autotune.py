@@ -38,6 +38,15 @@ GRID = {
 }
 
 
+def parse_grid(text):
+    """'waves_per_cu=4,8;copy_limit=0.5' -> grid dict (other keys from GRID)."""
+    grid = dict(GRID)
+    for part in filter(None, text.split(";")):
+        k, v = part.split("=")
+        grid[k] = [type(GRID[k][0])(x) for x in v.split(",")]
+    return grid
+
+
 def run_once(G, queries, budget, params, mem_budget):
     E = scs.Engine(G, node_budget=budget, mem_budget=mem_budget, **params)
     E.run([queries[0]])  # code objects and first-touch outside the timing
@@ -72,6 +81,7 @@ def main():
     ap.add_argument("--queries", type=int, default=512)
     ap.add_argument("--budget", type=int, default=300)
     ap.add_argument("--mem-gb", type=float, default=32.0)
+    ap.add_argument("--grid", default="", help="override grid axes, e.g. 'waves_per_cu=4,8'")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     g = configs.graph(args.config)
@@ -79,7 +89,8 @@ def main():
     core = G.core().astype(np.int64)
     qs = [(q.l, q.h, q.q, q.heu, q.limit_doms)
           for q in configs.queries(args.config, core, count=args.queries)]
-    best, rows = tune(G, qs, args.budget, mem_budget=int(args.mem_gb * (1 << 30)))
+    best, rows = tune(G, qs, args.budget, grid=parse_grid(args.grid),
+                      mem_budget=int(args.mem_gb * (1 << 30)))
     out = {"config": args.config, "queries": len(qs), "budget": args.budget, "best": best,
            "sweep": rows}
     print(json.dumps({"best": best}), flush=True)

@@ -347,7 +347,8 @@ extern "C" void scs_params_default(scs_params* p) {
   p->deterministic = 0;
   p->learned = 0;
   p->model = nullptr;
-  p->heuristic_threads = 4;
+  p->heuristic_threads = 8;
+  p->admit_per_step = 256;
 }
 
 static u64 align256(u64 x) { return (x + 255) & ~255ull; }
@@ -390,7 +391,8 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   if (p.waves_per_cu == 0) p.waves_per_cu = 16;
   if (p.waves_per_cu > 16) p.waves_per_cu = 16;  // LDS per wave (~8 KB) x 16 fits 160 KB
   if (p.time_slice_us == 0) p.time_slice_us = 2000;
-  if (p.heuristic_threads == 0) p.heuristic_threads = 4;
+  if (p.heuristic_threads == 0) p.heuristic_threads = 8;
+  if (p.admit_per_step == 0) p.admit_per_step = 256;
   scs_engine* E = new scs_engine;
   E->g = g;
   E->p = p;
@@ -409,6 +411,21 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   E->kcap = p.max_kept ? std::min(p.max_kept, n) : n;
   if (E->kcap == 0) E->kcap = 1;
   if (p.partition_size && p.partition_size < E->kcap) E->kcap = std::max<u32>(p.partition_size, 1);
+  if (!p.max_kept && !p.partition_size) {
+    // per-wave depth-first state grows with kcap: keep all waves' state
+    // within 60% of the device memory left after the arena and the batched
+    // preprocessing buffers; larger kept sets are rejected per query
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return fail(SCS_EHIP);
+    const u64 fixed = p.mem_budget + 420ull * n + (64ull << 20);
+    const u64 room = freeb > fixed ? (u64)((freeb - fixed) * 0.6) : 0;
+    const u64 pool_slots = std::max<u64>(p.limit_tasks ? p.limit_tasks : 2ull * E->nwaves,
+                                         (u64)E->nwaves + p.limit_queries + 64);
+    auto need = [&](u32 kc) {
+      return (u64)E->nwaves * make_layout(kc).stride + pool_slots * 4ull * ((kc + 1) / 2 + 1100);
+    };
+    while (E->kcap > 4096 && need(E->kcap) > room) E->kcap = E->kcap / 4 * 3;
+  }
   const WaveLayout lay = make_layout(E->kcap);
   // pool
   u32 donate_below = p.deterministic ? 0u : (u32)std::max(1.0, p.copy_limit * E->nwaves);
@@ -469,6 +486,7 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   A.donate_below = donate_below;
   A.deterministic = p.deterministic;
   A.node_flush = p.deterministic ? 1 : 16;
+  A.waves_per_cu = p.deterministic ? 1 : p.waves_per_cu;
   E->slots.resize(nq);
   E->h_slots.resize(nq);
   E->heu_work.resize(p.heuristic_threads);
@@ -856,7 +874,8 @@ extern "C" int scs_step(scs_engine* E) {
   std::vector<PendingQuery> batch;
   {
     std::lock_guard<std::mutex> lk(E->qmu);
-    while (!E->queue.empty() && batch.size() < free_slots.size()) {
+    while (!E->queue.empty() && batch.size() < free_slots.size() &&
+           batch.size() < E->p.admit_per_step) {
       batch.push_back(E->queue.front());
       E->queue.pop_front();
     }
@@ -1049,6 +1068,15 @@ extern "C" int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_
   if (hsize) *hsize = (u32)HH.size();
   if (H)
     for (u32 i = 0; i < HH.size() && i < cap; i++) H[i] = HH[i];
+  return SCS_OK;
+}
+
+extern "C" int scs_engine_heuristic(scs_engine* E, uint32_t l, uint32_t h, uint32_t q,
+                                    uint32_t* kl, uint32_t* ubD) {
+  if (!E || !kl || h == 0 || l > h || q >= E->g->hg.n) return SCS_EINVAL;
+  std::vector<u32> HH;
+  *kl = scs::cssc_heu(E->g->hg, l, h, q, E->heu_work[0], HH);
+  if (ubD) *ubD = scs::ubd_host(*kl, h);
   return SCS_OK;
 }
 

@@ -159,6 +159,13 @@ struct W {
   u32* trace_cnt;
   u32 trace_cap, trace_stride;
   EngineGlobals* gl;
+  // LDS bitmaps over the bound query's local ids (when they fit): xb = the
+  // vertices with status X, tb = scratch set (C during scoring, N[ustar]
+  // during the domination test), all-zero between uses
+  u32* xb;
+  u32* tb;
+  u32 bm;         // 1 = the bitmaps are in use for the bound query
+  u32 lds_words;  // dynamic LDS words of this wave
   // wave-uniform counters
   u32 nC, nR, ntrail, nframes, nsav;
   u64 edges;
@@ -189,6 +196,11 @@ __device__ void bind(W& w, const EngineArgs& A, u32 wave, Lds* L) {
   w.nsav = 0;
   w.model = A.model;
   w.gl = A.glob;
+  extern __shared__ u32 lds_dyn[];
+  w.lds_words = A.lds_words;
+  w.xb = lds_dyn;
+  w.tb = lds_dyn;
+  w.bm = 0;
   w.trace = A.trace;
   w.trace_cnt = A.trace_cnt;
   w.trace_cap = A.trace_cap;
@@ -209,6 +221,29 @@ __device__ void bind_query(W& w, QuerySlot* Q) {
   w.limit_doms = Q->limit_doms;
   w.dmax = Q->dmax;
   w.ku = Q->ku;
+  const u32 nw = (w.K + 31) / 32;
+  w.bm = 2 * nw <= w.lds_words ? 1u : 0u;
+  w.tb = w.xb + nw;
+}
+
+__device__ __forceinline__ bool xbit(const W& w, u32 v) { return (w.xb[v >> 5] >> (v & 31)) & 1u; }
+__device__ __forceinline__ bool tbit(const W& w, u32 v) { return (w.tb[v >> 5] >> (v & 31)) & 1u; }
+__device__ __forceinline__ void tset(W& w, u32 v) { atomicOr(&w.tb[v >> 5], 1u << (v & 31)); }
+__device__ __forceinline__ void tclr(W& w, u32 v) { atomicAnd(&w.tb[v >> 5], ~(1u << (v & 31))); }
+
+// Builds xb from the status array and clears tb (bitmap mode only).
+__device__ void build_bitmaps(W& w) {
+  if (!w.bm) return;
+  const u32 K = w.K;
+  const u32 nw = (K + 31) / 32;
+  for (u32 v0 = 0; v0 < K; v0 += 64) {
+    const u32 v = v0 + LANE;
+    const u64 m = __ballot(v < K && w.st[v] == ST_X);
+    if (LANE == 0) w.xb[v0 >> 5] = (u32)m;
+    if (LANE == 1 && (v0 >> 5) + 1 < nw) w.xb[(v0 >> 5) + 1] = (u32)(m >> 32);
+  }
+  for (u32 i = LANE; i < nw; i += 64) w.tb[i] = 0;
+  wsync();
 }
 
 // ---------------------------------------------------------------------------
@@ -341,8 +376,14 @@ struct Clo {
 };
 
 __device__ __forceinline__ void dec_neighbour(W& w, u32 x, const Clo& c) {
-  u8 s = w.st[x];
-  if (s == ST_X) return;
+  u8 s = ST_R;
+  if (w.bm) {
+    if (xbit(w, x)) return;
+    if (c.on) s = w.st[x];
+  } else {
+    s = w.st[x];
+    if (s == ST_X) return;
+  }
   u32 old = atomicSub(&w.dR[x], 1u);
   if (c.on && s == ST_R) {
     u32 dc = w.dC[x];
@@ -397,6 +438,7 @@ __device__ void remove_batch(W& w, const u32* list, u32 m, const Clo& c) {
   for (u32 i = LANE; i < m; i += 64) {
     u32 v = list[i];
     w.st[v] = ST_X;
+    if (w.bm) atomicOr(&w.xb[v >> 5], 1u << (v & 31));
     w.queued[v] = 0;
     atomicSub(&hist[w.dC[v]], 1u);
     w.trail[w.ntrail + i] = v;
@@ -462,6 +504,7 @@ __device__ void undo_batch(W& w, u32 start, u32 m, bool recount) {
   for (u32 i = LANE; i < m; i += 64) {
     u32 v = E[i];
     w.st[v] = ST_R;
+    if (w.bm) atomicAnd(&w.xb[v >> 5], ~(1u << (v & 31)));
     atomicAdd(&hist[w.dC[v]], 1u);
     w.rlist[w.nR + i] = v;
     w.pos[v] = w.nR + i;
@@ -473,7 +516,7 @@ __device__ void undo_batch(W& w, u32 start, u32 m, bool recount) {
     scan_adj<u32>(
         w, m, [&](u32 i) { return E[i]; },
         [&](u32 x) {
-          if (w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
+          if (w.bm ? !xbit(w, x) : w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
           return 0u;
         },
         [&](u32, u32, u32) {});
@@ -483,7 +526,7 @@ __device__ void undo_batch(W& w, u32 start, u32 m, bool recount) {
       const u32 b = w.off[v], e = w.off[v + 1];
       for (u32 j = b + LANE; j < e; j += 64) {
         u32 x = w.adj[j];
-        if (w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
+        if (w.bm ? !xbit(w, x) : w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
       }
       w.edges += e - b;
     }
@@ -831,19 +874,28 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   if (LANE == 0) L->nfront = 0;
   wsync();
   const bool single0 = (Cs == 1 && w.dC[w.clist[0]] == 0);
+  if (w.bm) {
+    // tb = C, so "x is R" is two LDS bit tests
+    for (u32 i = LANE; i < Cs; i += 64) tset(w, w.clist[i]);
+    wsync();
+  }
   for (u32 i = 0; i < Cs; i++) {
     u32 u = w.clist[i];
     u64 r = recip32(w.dC[u]);
     const u32 b = w.off[u], e = w.off[u + 1];
     for (u32 j = b + LANE; j < e; j += 64) {
       u32 x = w.adj[j];
-      if (w.st[x] != ST_R) continue;
+      if (w.bm ? (xbit(w, x) || tbit(w, x)) : w.st[x] != ST_R) continue;
       u64 old = atomicAdd((unsigned long long*)&w.acc[x], (unsigned long long)r);
       if (old == 0) w.front[atomicAdd(&L->nfront, 1u)] = x;
     }
     w.edges += e - b;
   }
   wsync();
+  if (w.bm) {
+    for (u32 i = LANE; i < Cs; i += 64) tclr(w, w.clist[i]);
+    wsync();
+  }
   const u32 nf = bcast(L->nfront);
   // argmax score, ties to the lowest id
   u64 best = 0;
@@ -875,7 +927,14 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   if ((w.flags & QF_DOMS) && w.limit_doms > 0) {
     const u32 lu = w.dC[us] + w.dR[us];
     const u32 ub0 = w.off[us], ue = w.off[us + 1];
-    for (u32 j = ub0 + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 1;
+    const bool bm = w.bm;
+    if (bm) {
+      // tb = N[us]: the test becomes two LDS bit lookups per entry
+      for (u32 j = ub0 + LANE; j < ue; j += 64) tset(w, w.adj[j]);
+      if (LANE == 0) tset(w, us);
+    } else {
+      for (u32 j = ub0 + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 1;
+    }
     if (LANE == 0) L->ncand = 0;
     wsync();
     // x is dominated when every live neighbour is us or a neighbour of us;
@@ -887,20 +946,36 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
       ntest++;
       bool dom = true;
       const u32 b = w.off[x], e = w.off[x + 1];
-      for (u32 j = b; j < e; j++) {
-        u32 y = w.adj[j];
-        ed++;
-        if (w.st[y] == ST_X) continue;
-        if (y != us && !w.nmark[y]) {
-          dom = false;
-          break;
+      if (bm) {
+        for (u32 j = b; j < e; j++) {
+          const u32 y = w.adj[j];
+          ed++;
+          if (!tbit(w, y) && !xbit(w, y)) {
+            dom = false;
+            break;
+          }
+        }
+      } else {
+        for (u32 j = b; j < e; j++) {
+          u32 y = w.adj[j];
+          ed++;
+          if (w.st[y] == ST_X) continue;
+          if (y != us && !w.nmark[y]) {
+            dom = false;
+            break;
+          }
         }
       }
       if (dom) w.work[atomicAdd(&L->ncand, 1u)] = x;
     }
     w.edges += wsum(ed) + (ue - ub0);
     wsync();
-    for (u32 j = ub0 + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 0;
+    if (bm) {
+      for (u32 j = ub0 + LANE; j < ue; j += 64) tclr(w, w.adj[j]);
+      if (LANE == 0) tclr(w, us);
+    } else {
+      for (u32 j = ub0 + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 0;
+    }
     const u32 nc = bcast(L->ncand);
     ntest = wsum(ntest);
     if (LANE == 0) {
@@ -1006,6 +1081,7 @@ __device__ void rebuild(W& w, const u32* task) {
   }
   for (u32 i = LANE; i < SCS_HCAP + 4; i += 64) w.L->hist[i] = 0;
   wsync();
+  build_bitmaps(w);
   // R list by ballot compaction (ascending ids)
   u32 base = 0;
   for (u32 v0 = 0; v0 < K; v0 += 64) {
@@ -1164,6 +1240,7 @@ __global__ void __launch_bounds__(64, 4) scs_epoch(EngineArgs A) {
     w.nsav = H->nsav;
     for (u32 i = LANE; i < SCS_HCAP + 4; i += 64) L.hist[i] = w.ghist[i];
     wsync();
+    build_bitmaps(w);
   }
   bool idle_counted = false;
   u32 idle_spins = 0;
@@ -1372,11 +1449,24 @@ __global__ void __launch_bounds__(64, 4) scs_epoch(EngineArgs A) {
   }
 }
 
+// Dynamic LDS of one wave: its share of the CU's 160 KB at the requested
+// waves per CU, minus the static part, capped at 48 KB.
+static u32 epoch_lds_bytes(u32 waves_per_cu) {
+  const u32 wpc = waves_per_cu ? waves_per_cu : 16;
+  const i64 share = (i64)(160 * 1024 / wpc) - (i64)sizeof(Lds);
+  i64 b = share < (i64)(48 * 1024) ? share : (i64)(48 * 1024);
+  b = b > 0 ? (b & ~(i64)511) : 0;
+  return (u32)b;
+}
+
 extern "C" hipError_t scs_launch_epoch(const EngineArgs* A, hipStream_t s) {
-  if (A->model)
-    scs_epoch<true><<<A->nwaves, 64, 0, s>>>(*A);
+  EngineArgs a = *A;
+  const u32 dyn = epoch_lds_bytes(a.waves_per_cu);
+  a.lds_words = dyn / 4;
+  if (a.model)
+    scs_epoch<true><<<a.nwaves, 64, dyn, s>>>(a);
   else
-    scs_epoch<false><<<A->nwaves, 64, 0, s>>>(*A);
+    scs_epoch<false><<<a.nwaves, 64, dyn, s>>>(a);
   return hipGetLastError();
 }
 

@@ -161,5 +161,7 @@ struct EngineArgs {
   u32 donate_below;    // donate while fewer tasks are pooled
   u32 deterministic;
   u32 node_flush;      // flush node counters every this many nodes
+  u32 waves_per_cu;    // resident waves per CU the launch is sized for
+  u32 lds_words;       // dynamic LDS words per wave (set by the launcher)
   u32 pad;
 };

@@ -45,7 +45,7 @@ class Params(C.Structure):
                 ("max_kept", C.c_uint32), ("time_slice_us", C.c_uint32),
                 ("deterministic", C.c_uint32), ("learned", C.c_uint32),
                 ("model", C.c_void_p), ("heuristic_threads", C.c_uint32),
-                ("reserved", C.c_uint32 * 7)]
+                ("admit_per_step", C.c_uint32), ("reserved", C.c_uint32 * 6)]
 
 
 class Result(C.Structure):
@@ -319,6 +319,30 @@ class Engine:
     def trace(self, capacity_rows, stride=1):
         _check(lib().scs_engine_trace(self._h, capacity_rows, stride), "scs_engine_trace")
         self._trace_cap = capacity_rows
+
+    def heuristic(self, l, h, q):
+        """(kl, ubD): the admission lower bound and BFS depth bound of a query."""
+        kl, ubd = C.c_uint32(), C.c_uint32()
+        _check(lib().scs_engine_heuristic(self._h, l, h, q, C.byref(kl), C.byref(ubd)),
+               "heuristic")
+        return kl.value, ubd.value
+
+    def prep_many(self, q, kl0, ubd):
+        """Batched preprocessing of up to 64 queries (local graphs built on the
+        device, then released); returns (K, nnz) per query."""
+        n = len(q)
+        q = np.ascontiguousarray(q, np.uint32)
+        kl0 = np.ascontiguousarray(kl0, np.uint32)
+        ubd = np.ascontiguousarray(ubd, np.uint32)
+        K = np.zeros(n, np.uint32)
+        nnz = np.zeros(n, np.uint64)
+        f = lib().scs_prep_probe_many
+        u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+        f.argtypes = [C.c_void_p, C.c_uint32, u32p, u32p, u32p, u32p,
+                      np.ctypeslib.ndpointer(np.uint64, flags="C"), C.c_void_p, C.c_void_p,
+                      C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]
+        _check(f(self._h, n, q, kl0, ubd, K, nnz, None, None, None, 0, None, 0), "prep_many")
+        return K, nnz
 
     def trace_read(self):
         cap = getattr(self, "_trace_cap", 0)
