@@ -58,9 +58,9 @@ def main():
     ap.add_argument("--budget", type=int, default=300, help="node budget per query")
     # defaults: the autotuner's pick for this workload (scs_amd/tuned_c2.json)
     ap.add_argument("--heuristic-threads", type=int, default=12)
-    ap.add_argument("--waves-per-cu", type=int, default=8)
+    ap.add_argument("--waves-per-cu", type=int, default=4)
     ap.add_argument("--limit-queries", type=int, default=2048)
-    ap.add_argument("--slice-us", type=int, default=5000)
+    ap.add_argument("--slice-us", type=int, default=200000)
     ap.add_argument("--copy-limit", type=float, default=0.5)
     ap.add_argument("--learned", type=int, default=0,
                     help="1 = the main leg itself uses the learned order")

@@ -226,10 +226,10 @@ __device__ void bind_query(W& w, QuerySlot* Q) {
   w.tb = w.xb + nw;
 }
 
-__device__ __forceinline__ bool xbit(const W& w, u32 v) { return (w.xb[v >> 5] >> (v & 31)) & 1u; }
-__device__ __forceinline__ bool tbit(const W& w, u32 v) { return (w.tb[v >> 5] >> (v & 31)) & 1u; }
-__device__ __forceinline__ void tset(W& w, u32 v) { atomicOr(&w.tb[v >> 5], 1u << (v & 31)); }
-__device__ __forceinline__ void tclr(W& w, u32 v) { atomicAnd(&w.tb[v >> 5], ~(1u << (v & 31))); }
+__device__ bool xbit(const W& w, u32 v) { return (w.xb[v >> 5] >> (v & 31)) & 1u; }
+__device__ bool tbit(const W& w, u32 v) { return (w.tb[v >> 5] >> (v & 31)) & 1u; }
+__device__ void tset(W& w, u32 v) { atomicOr(&w.tb[v >> 5], 1u << (v & 31)); }
+__device__ void tclr(W& w, u32 v) { atomicAnd(&w.tb[v >> 5], ~(1u << (v & 31))); }
 
 // Builds xb from the status array and clears tb (bitmap mode only).
 __device__ void build_bitmaps(W& w) {
@@ -375,7 +375,7 @@ struct Clo {
   u32* cnt;  // LDS counter of next
 };
 
-__device__ __forceinline__ void dec_neighbour(W& w, u32 x, const Clo& c) {
+__device__ void dec_neighbour(W& w, u32 x, const Clo& c) {
   u8 s = ST_R;
   if (w.bm) {
     if (xbit(w, x)) return;
@@ -591,19 +591,19 @@ __device__ void offer(W& w, u32 mn) {
   wsync();
 }
 
-__device__ __forceinline__ u32 cur_kl(const W& w) { return aload(&w.Q->kl); }
+__device__ u32 cur_kl(const W& w) { return aload(&w.Q->kl); }
 
 // ---------------------------------------------------------------------------
 // Node processing.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool cond12(const W& w, u32 v, u32 Cs, u32 ubD, u32 kl) {
+__device__ bool cond12(const W& w, u32 v, u32 Cs, u32 ubD, u32 kl) {
   u32 dc = w.dC[v], dr = w.dR[v];
   if ((w.flags & QF_RED1) && min(dr + dc, dc + w.h - Cs - 1) <= kl) return true;
   if ((w.flags & QF_RED2) && ubD < w.dist[v]) return true;
   return false;
 }
 
-__device__ __forceinline__ void try_push(W& w, u32 x, u32 Cs, u32 ubD, u32 kl, u32* list,
+__device__ void try_push(W& w, u32 x, u32 Cs, u32 ubD, u32 kl, u32* list,
                                          u32* cnt) {
   if (w.st[x] == ST_R && !w.queued[x] && cond12(w, x, Cs, ubD, kl)) {
     u32* wp = (u32*)((uintptr_t)(w.queued + x) & ~(uintptr_t)3);
@@ -788,6 +788,20 @@ __device__ void trace_node(W& w, u32 nf, u32 Cs, u32 kl) {
   }
 }
 
+// Zeroes acc[] of the frontier (four independent entries per lane per step).
+__device__ void clear_acc(W& w, u32 nf) {
+  u32 i = LANE;
+  for (; i + 192 < nf; i += 256) {
+    const u32 x0 = w.front[i], x1 = w.front[i + 64], x2 = w.front[i + 128], x3 = w.front[i + 192];
+    w.acc[x0] = 0;
+    w.acc[x1] = 0;
+    w.acc[x2] = 0;
+    w.acc[x3] = 0;
+  }
+  for (; i < nf; i += 64) w.acc[w.front[i]] = 0;
+  wsync();
+}
+
 // LEARN selects the kernel variant that can score candidates with the MLP;
 // the plain variant leaves that path out and needs half the registers.
 template <bool LEARN>
@@ -900,12 +914,29 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   // argmax score, ties to the lowest id
   u64 best = 0;
   u32 bid = INV;
-  for (u32 i = LANE; i < nf; i += 64) {
-    u32 x = w.front[i];
-    u64 s = w.acc[x] + rterm(w.dR[x], w.dmax);
+  auto consider = [&](u32 x, u64 s) {
     if (s > best || (s == best && x < bid)) {
       best = s;
       bid = x;
+    }
+  };
+  {
+    u32 i = LANE;
+    // four candidates per lane per step: their gathers overlap (the
+    // learned variant has no registers to spare for it)
+    for (; !LEARN && i + 192 < nf; i += 256) {
+      const u32 x0 = w.front[i], x1 = w.front[i + 64], x2 = w.front[i + 128],
+                x3 = w.front[i + 192];
+      const u64 a0 = w.acc[x0], a1 = w.acc[x1], a2 = w.acc[x2], a3 = w.acc[x3];
+      const u32 r0 = w.dR[x0], r1 = w.dR[x1], r2 = w.dR[x2], r3 = w.dR[x3];
+      consider(x0, a0 + rterm(r0, w.dmax));
+      consider(x1, a1 + rterm(r1, w.dmax));
+      consider(x2, a2 + rterm(r2, w.dmax));
+      consider(x3, a3 + rterm(r3, w.dmax));
+    }
+    for (; i < nf; i += 64) {
+      const u32 x = w.front[i];
+      consider(x, w.acc[x] + rterm(w.dR[x], w.dmax));
     }
   }
   const u64 gbest = wmax64(best);
@@ -917,8 +948,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     if (ul != INV) us = ul;
   }
   if (nf == 0 || us == INV) {
-    for (u32 i = LANE; i < nf; i += 64) w.acc[w.front[i]] = 0;
-    wsync();
+    clear_acc(w, nf);
     return o;
   }
   // dominated set of us
@@ -947,13 +977,17 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
       bool dom = true;
       const u32 b = w.off[x], e = w.off[x + 1];
       if (bm) {
-        for (u32 j = b; j < e; j++) {
-          const u32 y = w.adj[j];
-          ed++;
-          if (!tbit(w, y) && !xbit(w, y)) {
-            dom = false;
-            break;
-          }
+        // four entries per step (one cache line mostly); us is in tb, so a
+        // padding entry of us always passes
+        for (u32 j = b; j < e && dom; j += 4) {
+          const u32 y0 = w.adj[j];
+          const u32 y1 = j + 1 < e ? w.adj[j + 1] : us;
+          const u32 y2 = j + 2 < e ? w.adj[j + 2] : us;
+          const u32 y3 = j + 3 < e ? w.adj[j + 3] : us;
+          ed += min(4u, e - j);
+          const bool ok = (tbit(w, y0) || xbit(w, y0)) && (tbit(w, y1) || xbit(w, y1)) &&
+                          (tbit(w, y2) || xbit(w, y2)) && (tbit(w, y3) || xbit(w, y3));
+          if (!ok) dom = false;
         }
       } else {
         for (u32 j = b; j < e; j++) {
@@ -1008,8 +1042,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     }
     nD = want;
   }
-  for (u32 i = LANE; i < nf; i += 64) w.acc[w.front[i]] = 0;
-  wsync();
+  clear_acc(w, nf);
   o.branch = 1;
   o.ustar = us;
   o.nD = nD;

@@ -100,7 +100,21 @@ __global__ void ms_seed(const MsBatch* __restrict__ B, u64* __restrict__ vis,
 }
 
 // One BFS level for every query whose depth bound exceeds `level` (act).
-// Runs only if the previous level found something (flag_prev).
+// Runs only if the previous level found something (flag_prev).  A 16-lane
+// segment takes one vertex (four per wave) and reads 16 adjacency entries
+// per step, stopping once it holds every bit it wants; vertices of degree
+// above MS_HUB are taken by the whole wave.
+#define MS_HUB 512
+
+__device__ __forceinline__ u64 ms_seg_or(u64 x) {
+  u32 lo = (u32)x, hi = (u32)(x >> 32);
+  for (int o = 8; o > 0; o >>= 1) {
+    lo |= __shfl_xor(lo, o, 16);
+    hi |= __shfl_xor(hi, o, 16);
+  }
+  return ((u64)hi << 32) | lo;
+}
+
 __global__ void __launch_bounds__(256) ms_level(const i64* __restrict__ off,
                                                 const int32_t* __restrict__ adj, u32 n, u32 level,
                                                 u64 act, const u64* __restrict__ elig,
@@ -110,67 +124,77 @@ __global__ void __launch_bounds__(256) ms_level(const i64* __restrict__ off,
                                                 u32* __restrict__ flag_cur,
                                                 unsigned long long* bytes) {
   if (level > 0 && *flag_prev == 0) return;
-  const u32 lane = threadIdx.x & 63;
+  const u32 lane = threadIdx.x & 63, seg = lane >> 4, sl = lane & 15;
   const u64 wid = (blockIdx.x * (u64)blockDim.x + threadIdx.x) >> 6;
   const u64 nw = ((u64)gridDim.x * blockDim.x) >> 6;
   u64 nbytes = 0;
   bool found = false;
-  for (u64 base = wid * 64; base < n; base += nw * 64) {
-    const u64 vv = base + lane;
+  for (u64 base = wid * 4; base < n; base += nw * 4) {
+    const u64 vv = base + seg;
     const bool in = vv < n;
     const u32 v = (u32)vv;
-    u64 want = 0, acc = 0, visv = 0;
+    u64 want = 0, visv = 0;
     i64 b0 = 0, e0 = 0;
     if (in) {
       visv = vis[v];
       want = elig[v] & act & ~visv;
-      nbytes += 16;
+      if (sl == 0) nbytes += 16;
     }
     if (want) {
       b0 = off[v];
       e0 = off[v + 1];
-      nbytes += 16;
+      if (sl == 0) nbytes += 16;
     }
-    const bool big = want && (e0 - b0) > 64;
-    if (want && !big) {
-      for (i64 j = b0; j < e0; j++) {
-        acc |= fr[(u32)adj[j]] & want;
+    const bool big = want && (e0 - b0) > MS_HUB;
+    bool active = want && !big;
+    u64 acc = 0;
+    i64 j = b0 + sl;
+    while (__ballot(active)) {
+      u64 x = 0;
+      if (active && j < e0) {
+        x = fr[(u32)adj[j]] & want;
         nbytes += 12;
-        if (acc == want) break;
       }
+      acc |= ms_seg_or(x);
+      j += 16;
+      if (active && (acc == want || j - (i64)sl >= e0)) active = false;
     }
-    u64 bm = __ballot(big);
+    // hubs: the whole wave, 64 entries per step
+    u64 bm = __ballot(big && sl == 0);
     while (bm) {
       const int src = __ffsll((unsigned long long)bm) - 1;
       bm &= bm - 1;
       const u64 bwant = ms_shfl64(want, src);
       const i64 bb = (i64)ms_shfl64((u64)b0, src), be = (i64)ms_shfl64((u64)e0, src);
       u64 bacc = 0;
-      for (i64 j = bb; j < be; j += 64) {
+      for (i64 jj = bb; jj < be; jj += 64) {
         u64 x = 0;
-        if (j + lane < be) {
-          x = fr[(u32)adj[j + lane]] & bwant;
+        if (jj + lane < be) {
+          x = fr[(u32)adj[jj + lane]] & bwant;
           nbytes += 12;
         }
         bacc |= ms_or(x);
         if (bacc == bwant) break;
       }
-      if ((int)lane == src) acc = bacc;
+      // hand the result to the hub's segment
+      if ((lane >> 4) == ((u32)src >> 4)) acc = bacc;
     }
     if (in) {
-      nf[v] = acc;
-      nbytes += 8;
-      if (acc) {
-        vis[v] = visv | acc;
-        found = true;
+      if (sl == 0) {
+        nf[v] = acc;
         nbytes += 8;
-        u64 m = acc;
-        while (m) {
-          const int b = __ffsll((unsigned long long)m) - 1;
-          m &= m - 1;
-          dist16[(u64)v * MS_BITS + b] = (u16)(level + 1);
-          nbytes += 2;
+      }
+      if (acc) {
+        if (sl == 0) {
+          vis[v] = visv | acc;
+          nbytes += 8;
         }
+        found = true;
+        for (u32 b = sl; b < 64; b += 16)
+          if ((acc >> b) & 1ull) {
+            dist16[(u64)v * MS_BITS + b] = (u16)(level + 1);
+            nbytes += 2;
+          }
       }
     }
   }
@@ -435,8 +459,8 @@ extern "C" hipError_t scs_launch_ms_level(const i64* off, const int32_t* adj, u3
                                           u64* nf, u16* dist16, const u32* flag_prev,
                                           u32* flag_cur, unsigned long long* bytes,
                                           hipStream_t s) {
-  ms_level<<<ms_blocks(n, 16384), 256, 0, s>>>(off, adj, n, level, act, elig, vis, fr, nf, dist16,
-                                                flag_prev, flag_cur, bytes);
+  ms_level<<<ms_blocks(((u64)n + 3) / 4 * 64, 16384), 256, 0, s>>>(
+      off, adj, n, level, act, elig, vis, fr, nf, dist16, flag_prev, flag_cur, bytes);
   return hipGetLastError();
 }
 extern "C" hipError_t scs_launch_ms_count(u32 n, u32 G, const u64* vis, u64 valid, u32* gcnt,
