@@ -178,6 +178,19 @@ def main():
         "heuristic_host_us": round(s1["heu_us"] - s0["heu_us"], 1),
         "core_decomposition_us": round(s1["core_us"], 1),
     }
+    # HBM traffic per launch from the committed hardware-counter profile of
+    # this workload and engine setting (two rocprofv3 --pmc runs, FETCH_SIZE
+    # and WRITE_SIZE, scaled by a copy pass of known size)
+    prof = os.path.join(ROOT, "profiles", "scs_epoch_c2_traffic.json")
+    if os.path.exists(prof):
+        t = json.load(open(prof))
+        if (t["workload"] == args.config and t["waves_per_cu"] == args.waves_per_cu and
+                t["time_slice_us"] == args.slice_us and t["node_budget"] == args.budget and
+                not args.learned):
+            line["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
+            line["roofline"]["traffic_source"] = "profiles/scs_epoch_c2_traffic.json"
+            line["roofline"]["algorithmic_bytes_per_launch"] = int(
+                edges * BYTES_PER_EDGE / max(1, epochs))
     line["kl_sum"] = int(sum(r.kl for r in all_res))
     line["kcap"] = int(s1["kcap"])
     E.close()
