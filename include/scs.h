@@ -81,12 +81,14 @@ typedef struct scs_params {
   uint32_t red1, red2, red3;  /* argv[8..10]: reduction rules 1-3 */
   uint32_t prun1, prun2;      /* argv[11..12]: bounds U_d and U_nr */
   /* named extensions */
-  uint32_t waves_per_cu;   /* resident search waves per CU (0 = 16, max 16) */
+  uint32_t waves_per_cu;   /* resident search waves per CU (0 = 4, max 16); at 4 each
+                              wave has room for LDS bitmaps over 130K kept vertices */
   uint32_t limit_tasks;    /* task pool slots (0 = 4 * waves) */
   uint64_t node_budget;    /* per-query node cap before CAPPED (0 = none) */
   uint64_t mem_budget;     /* bytes of device memory for per-query graphs */
   uint32_t max_kept;       /* per-query kept-set cap (0 = graph size) */
-  uint32_t time_slice_us;  /* length of one scs_step kernel epoch */
+  uint32_t time_slice_us;  /* length of one scs_step kernel epoch (0 = 200 ms; a node
+                              in progress finishes after the deadline) */
   uint32_t deterministic;  /* 1 = a single wave and no work sharing: node
                               counts equal the CPU oracle's */
   uint32_t learned;        /* 1 = MLP scores the branching vertex */

@@ -338,12 +338,12 @@ extern "C" void scs_params_default(scs_params* p) {
   p->factor = 1;
   p->red1 = p->red2 = p->red3 = 1;
   p->prun1 = p->prun2 = 1;
-  p->waves_per_cu = 16;
+  p->waves_per_cu = 4;
   p->limit_tasks = 0;
   p->node_budget = 0;
   p->mem_budget = 1ull << 30;
   p->max_kept = 0;
-  p->time_slice_us = 2000;
+  p->time_slice_us = 200000;
   p->deterministic = 0;
   p->learned = 0;
   p->model = nullptr;
@@ -388,9 +388,9 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   else
     scs_params_default(&p);
   if (p.limit_queries == 0) p.limit_queries = 1;
-  if (p.waves_per_cu == 0) p.waves_per_cu = 16;
+  if (p.waves_per_cu == 0) p.waves_per_cu = 4;
   if (p.waves_per_cu > 16) p.waves_per_cu = 16;  // LDS per wave (~8 KB) x 16 fits 160 KB
-  if (p.time_slice_us == 0) p.time_slice_us = 2000;
+  if (p.time_slice_us == 0) p.time_slice_us = 200000;
   if (p.heuristic_threads == 0) p.heuristic_threads = 8;
   if (p.admit_per_step == 0) p.admit_per_step = 256;
   scs_engine* E = new scs_engine;

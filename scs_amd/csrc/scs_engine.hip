@@ -277,7 +277,16 @@ __device__ void scan_adj(W& w, u32 m, VertexF vertex, EdgeF edge, DoneF done) {
     const bool big = i < m && (e - b) > SEQ_DEG;
     if (i < m && !big) {
       AccT acc = 0;
-      for (u32 j = b; j < e; j++) acc += edge(w.adj[j]);
+      u32 j = b;
+      // four entries per step: their gathers are in flight together
+      for (; j + 4 <= e; j += 4) {
+        const u32 x0 = w.adj[j], x1 = w.adj[j + 1], x2 = w.adj[j + 2], x3 = w.adj[j + 3];
+        acc += edge(x0);
+        acc += edge(x1);
+        acc += edge(x2);
+        acc += edge(x3);
+      }
+      for (; j < e; j++) acc += edge(w.adj[j]);
       done(i, v, acc);
       ed += e - b;
     }
@@ -404,7 +413,7 @@ __device__ void dec_neighbour(W& w, u32 x, const Clo& c) {
 // touching its neighbours and then recounts dR of the surviving live
 // vertices (cost: their total degree), pushing the old values on the save
 // stack for the undo.  Both leave identical dC/dR on every live vertex.
-__device__ void remove_batch(W& w, const u32* list, u32 m, const Clo& c) {
+__device__ __forceinline__ void remove_batch(W& w, const u32* list, u32 m, const Clo& c) {
   if (m == 0) return;
   u32* hist = w.L->hist;
   const u32 live_after = w.nC + (w.nR - m);
@@ -619,7 +628,7 @@ __device__ void try_push(W& w, u32 x, u32 Cs, u32 ubD, u32 kl, u32* list,
 // (|C| grew), or everything when kl changed.  Each round removes all
 // queued vertices as one batch and queues the neighbours it pushes below
 // the rule-1 threshold.  Returns true if anything was removed.
-__device__ bool r12(W& w, u32 from, bool full, u32 kl) {
+__device__ __forceinline__ bool r12(W& w, u32 from, bool full, u32 kl) {
   Lds* L = w.L;
   const u32 Cs = w.nC;
   const u32 ubD = ubd_node(kl, w.h);
@@ -1096,7 +1105,7 @@ __device__ int pool_try_dequeue(const Pool& P, u64* out_pos) {
 }
 
 // Rebuild the wave's state from a task payload.
-__device__ void rebuild(W& w, const u32* task) {
+__device__ __forceinline__ void rebuild(W& w, const u32* task) {
   const u32 nC = task[1], nL = task[2], form = task[3];
   const u32* C = task + 4;
   const u32* Lst = task + 4 + nC;
@@ -1131,8 +1140,56 @@ __device__ void rebuild(W& w, const u32* task) {
   w.nR = base;
   w.nC = nC;
   wsync();
-  // degrees of live vertices (dC in the high word, dR in the low word)
   const u32 nR = w.nR;
+  if (!form) {
+    // the list holds X (smaller than the live set): start every counter
+    // from the local degree (one coalesced pass) and subtract what X and C
+    // take away.  dR(v) = deg(v) - |N(v) & X| - |N(v) & C|, dC(v) = |N(v) & C|;
+    // counters of X vertices are never read, so their values do not matter.
+    for (u32 v = LANE; v < K; v += 64) {
+      w.dC[v] = 0;
+      w.dR[v] = w.off[v + 1] - w.off[v];
+    }
+    wsync();
+    if (nL >= 8) {
+      scan_adj<u32>(
+          w, nL, [&](u32 i) { return Lst[i]; },
+          [&](u32 y) {
+            atomicSub(&w.dR[y], 1u);
+            return 0u;
+          },
+          [&](u32, u32, u32) {});
+    } else {
+      for (u32 i = 0; i < nL; i++) {
+        const u32 x = Lst[i], b = w.off[x], e = w.off[x + 1];
+        for (u32 j = b + LANE; j < e; j += 64) atomicSub(&w.dR[w.adj[j]], 1u);
+        w.edges += e - b;
+      }
+    }
+    wsync();
+    // C one vertex at a time: its neighbours are distinct, so plain updates
+    for (u32 i = 0; i < nC; i++) {
+      const u32 c = C[i], b = w.off[c], e = w.off[c + 1];
+      for (u32 j = b + LANE; j < e; j += 64) {
+        const u32 y = w.adj[j];
+        w.dC[y] = (u16)(w.dC[y] + 1);
+        w.dR[y] -= 1;
+      }
+      w.edges += e - b;
+      wsync();
+    }
+    for (u32 i = LANE; i < nR; i += 64) {
+      const u32 v = w.rlist[i];
+      atomicAdd(&w.L->hist[min((u32)w.dC[v], w.h + 1)], 1u);
+    }
+    w.ntrail = 0;
+    w.nframes = 0;
+    w.nsav = 0;
+    wsync();
+    return;
+  }
+  // the list holds R: count the live vertices' neighbours directly
+  // (dC in the high word, dR in the low word)
   scan_adj<u64>(
       w, nR + nC, [&](u32 i) { return i < nR ? w.rlist[i] : w.clist[i - nR]; },
       [&](u32 x) {
@@ -1154,7 +1211,7 @@ __device__ void rebuild(W& w, const u32* task) {
 // Serialises the Z child of frame fi (the unexplored "exclude u*" branch)
 // into dst (dst[3] = form set by the caller: 1 = list R, 0 = list X);
 // returns the payload length in words.
-__device__ u32 serialize_z(W& w, u32 fi, u32* dst, u32 qslot) {
+__device__ __forceinline__ u32 serialize_z(W& w, u32 fi, u32* dst, u32 qslot) {
   const Frame& f = w.frames[fi];
   // mark vertices changed after f.mark (they were R at the frame)
   for (u32 t = f.mark + LANE; t < w.ntrail; t += 64) {
@@ -1201,7 +1258,7 @@ __device__ u32 serialize_z(W& w, u32 fi, u32* dst, u32 qslot) {
   return 4 + f.csize + base;
 }
 
-__device__ void save_state(W& w, u32 active, u32 qslot, u32 from, u32 kl_seen) {
+__device__ __forceinline__ void save_state(W& w, u32 active, u32 qslot, u32 from, u32 kl_seen) {
   if (active)
     for (u32 i = LANE; i < SCS_HCAP + 4; i += 64) w.ghist[i] = w.L->hist[i];
   if (LANE == 0) {
@@ -1248,8 +1305,10 @@ __device__ void flush_counters(W& w, const EngineArgs& A, u32& pend) {
 // ---------------------------------------------------------------------------
 // The persistent epoch kernel.
 // ---------------------------------------------------------------------------
+// flatten: every call is inlined, so the wave state W never needs an address
+// in private (scratch) memory
 template <bool LEARN>
-__global__ void __launch_bounds__(64, 4) scs_epoch(EngineArgs A) {
+__global__ void __launch_bounds__(64, 4) __attribute__((flatten)) scs_epoch(EngineArgs A) {
   __shared__ Lds L;
   const u32 wave = blockIdx.x;
   if (wave >= A.nwaves) return;

@@ -103,14 +103,16 @@ def main():
     ap.add_argument("--eval-queries", type=int, default=100)
     ap.add_argument("--out", default=os.path.join(ROOT, "models", "branching_c1.npz"))
     ap.add_argument("--report", default=None)
+    ap.add_argument("--limit-queries", type=int, default=1024)
     args = ap.parse_args()
 
+    print(f"[learn] building {args.config}", file=sys.stderr, flush=True)
     g = configs.graph(args.config)
     G = scs.Graph.from_csr(g.off, g.adj)
     core = G.core().astype(np.int64)
     tq = training_queries(args.config, core, per_seed=args.per_seed)
     t0 = time.perf_counter()
-    X, y, _ = collect(G, tq, args.budget, stride=args.stride, limit_queries=256)
+    X, y, _ = collect(G, tq, args.budget, stride=args.stride, limit_queries=args.limit_queries)
     t_col = time.perf_counter() - t0
     print(f"[learn] {len(tq)} training queries -> {len(y)} rows, {int(y.sum())} positive "
           f"({t_col:.1f} s)", file=sys.stderr, flush=True)
@@ -123,7 +125,7 @@ def main():
     m.save(args.out)
     eq = [(q.l, q.h, q.q, q.heu, q.limit_doms)
           for q in configs.queries(args.config, core, count=args.eval_queries)]
-    ev = evaluate(G, eq, m, args.budget, limit_queries=256)
+    ev = evaluate(G, eq, m, args.budget, limit_queries=args.limit_queries)
     same = all(a == b for a, b, s1, s2 in zip(ev["plain"]["kl"], ev["learned"]["kl"],
                                                  ev["plain"]["status"], ev["learned"]["status"])
                if s1 != "CAPPED" and s2 != "CAPPED")

@@ -54,10 +54,11 @@ def main():
     ap.add_argument("--budget", type=int, default=300)
     ap.add_argument("--model", default=None, help="learned-order model (.npz)")
     ap.add_argument("--limit-queries", type=int, default=256)
-    ap.add_argument("--slice-us", type=int, default=2000)
+    ap.add_argument("--slice-us", type=int, default=50000)
     ap.add_argument("--mem-gb", type=float, default=16.0, help="device arena for local graphs")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
+    print(f"[serve] building {args.config}", file=sys.stderr, flush=True)
     g = configs.graph(args.config)
     G = scs.Graph.from_csr(g.off, g.adj)
     core = G.core().astype(np.int64)
