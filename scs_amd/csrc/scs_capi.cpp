@@ -357,11 +357,8 @@ static WaveLayout make_layout(u32 kcap) {
   WaveLayout L{};
   u64 o = 0;
   const u64 k4 = (u64)((kcap + 3) & ~3u);
-  L.st = o; o = align256(o + k4);
+  L.vs = o; o = align256(o + sizeof(VState) * k4);
   L.nmark = o; o = align256(o + k4);
-  L.queued = o; o = align256(o + k4);
-  L.dC = o; o = align256(o + 2 * k4);
-  L.dR = o; o = align256(o + 4 * k4);
   L.pos = o; o = align256(o + 4 * k4);
   L.rlist = o; o = align256(o + 4 * k4);
   L.acc = o; o = align256(o + 8 * k4);

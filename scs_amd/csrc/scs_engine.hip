@@ -127,11 +127,8 @@ __device__ __forceinline__ void ph_switch(Lds* L, u64& tp, u32& ph, u32 next) {
 
 // ---------------------------------------------------------------------------
 struct W {
-  u8* st;
+  VState* vs;  // per vertex: dR, dC, status, queued flag (one 8-byte record)
   u8* nmark;
-  u8* queued;
-  u16* dC;
-  u32* dR;
   u32* pos;
   u32* rlist;
   u64* acc;
@@ -173,11 +170,8 @@ struct W {
 
 __device__ void bind(W& w, const EngineArgs& A, u32 wave, Lds* L) {
   u8* base = A.wave_mem + (u64)wave * A.lay.stride;
-  w.st = base + A.lay.st;
+  w.vs = (VState*)(base + A.lay.vs);
   w.nmark = base + A.lay.nmark;
-  w.queued = base + A.lay.queued;
-  w.dC = (u16*)(base + A.lay.dC);
-  w.dR = (u32*)(base + A.lay.dR);
   w.pos = (u32*)(base + A.lay.pos);
   w.rlist = (u32*)(base + A.lay.rlist);
   w.acc = (u64*)(base + A.lay.acc);
@@ -238,7 +232,7 @@ __device__ void build_bitmaps(W& w) {
   const u32 nw = (K + 31) / 32;
   for (u32 v0 = 0; v0 < K; v0 += 64) {
     const u32 v = v0 + LANE;
-    const u64 m = __ballot(v < K && w.st[v] == ST_X);
+    const u64 m = __ballot(v < K && w.vs[v].st == ST_X);
     if (LANE == 0) w.xb[v0 >> 5] = (u32)m;
     if (LANE == 1 && (v0 >> 5) + 1 < nw) w.xb[(v0 >> 5) + 1] = (u32)(m >> 32);
   }
@@ -307,6 +301,18 @@ __device__ void scan_adj(W& w, u32 m, VertexF vertex, EdgeF edge, DoneF done) {
   w.edges += wsum(ed);
 }
 
+// Adds delta to hist[d] for every lane with `on` (call on all lanes).  Most
+// updates hit dC = 0 or 1, so those two are summed over the wave first: one
+// LDS atomic instead of up to 64 colliding ones.
+__device__ __forceinline__ void hist_add(u32* hist, bool on, u32 d, int delta) {
+  const u64 m0 = __ballot(on && d == 0), m1 = __ballot(on && d == 1);
+  if (LANE == 0) {
+    if (m0) atomicAdd(&hist[0], (u32)(delta * (int)__popcll(m0)));
+    if (m1) atomicAdd(&hist[1], (u32)(delta * (int)__popcll(m1)));
+  }
+  if (on && d > 1) atomicAdd(&hist[d], (u32)delta);
+}
+
 // ---------------------------------------------------------------------------
 // State updates.  Counters of X vertices are frozen: a move or a removal
 // updates only the neighbours that are live at that moment, and an undo,
@@ -318,23 +324,32 @@ __device__ void scan_adj(W& w, u32 m, VertexF vertex, EdgeF edge, DoneF done) {
 __device__ void move_c(W& w, u32 v) {
   const u32 b = w.off[v], e = w.off[v + 1];
   u32* hist = w.L->hist;
-  for (u32 j = b + LANE; j < e; j += 64) {
-    u32 x = w.adj[j];
-    u8 s = w.st[x];
-    if (s == ST_X) continue;
-    u32 dc = w.dC[x];
-    if (s == ST_R) {
-      atomicSub(&hist[dc], 1u);
-      atomicAdd(&hist[dc + 1], 1u);
+  for (u32 j = b + LANE; j < e; j += 256) {
+    u32 x[4];
+    VState vs[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = j + 64 * k < e ? w.adj[j + 64 * k] : INV;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (x[k] != INV) vs[k] = w.vs[x[k]];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const bool live = x[k] != INV && vs[k].st != ST_X;
+      const u32 dc = live ? vs[k].dC : 0u;
+      const bool r = live && vs[k].st == ST_R;
+      hist_add(hist, r, dc, -1);
+      hist_add(hist, r, dc + 1, 1);
+      if (live) {
+        w.vs[x[k]].dC = (u16)(dc + 1);
+        w.vs[x[k]].dR = vs[k].dR - 1;
+      }
     }
-    w.dC[x] = (u16)(dc + 1);
-    w.dR[x] -= 1;
   }
   w.edges += e - b;
   wsync();
   if (LANE == 0) {
-    w.st[v] = ST_C;
-    hist[w.dC[v]] -= 1;
+    w.vs[v].st = ST_C;
+    hist[w.vs[v].dC] -= 1;
     u32 p = w.pos[v];
     u32 last = w.rlist[w.nR - 1];
     w.rlist[p] = last;
@@ -351,8 +366,8 @@ __device__ void move_c(W& w, u32 v) {
 __device__ void undo_move(W& w, u32 v) {
   u32* hist = w.L->hist;
   if (LANE == 0) {
-    w.st[v] = ST_R;
-    hist[w.dC[v]] += 1;
+    w.vs[v].st = ST_R;
+    hist[w.vs[v].dC] += 1;
     w.rlist[w.nR] = v;
     w.pos[v] = w.nR;
   }
@@ -360,17 +375,26 @@ __device__ void undo_move(W& w, u32 v) {
   w.nC--;
   wsync();
   const u32 b = w.off[v], en = w.off[v + 1];
-  for (u32 j = b + LANE; j < en; j += 64) {
-    u32 x = w.adj[j];
-    u8 s = w.st[x];
-    if (s == ST_X) continue;
-    u32 dc = w.dC[x];
-    if (s == ST_R) {
-      atomicSub(&hist[dc], 1u);
-      atomicAdd(&hist[dc - 1], 1u);
+  for (u32 j = b + LANE; j < en; j += 256) {
+    u32 x[4];
+    VState vs[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = j + 64 * k < en ? w.adj[j + 64 * k] : INV;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (x[k] != INV) vs[k] = w.vs[x[k]];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const bool live = x[k] != INV && vs[k].st != ST_X;
+      const u32 dc = live ? vs[k].dC : 1u;
+      const bool r = live && vs[k].st == ST_R;
+      hist_add(hist, r, dc, -1);
+      hist_add(hist, r, dc - 1, 1);
+      if (live) {
+        w.vs[x[k]].dC = (u16)(dc - 1);
+        w.vs[x[k]].dR = vs[k].dR + 1;
+      }
     }
-    w.dC[x] = (u16)(dc - 1);
-    w.dR[x] += 1;
   }
   w.edges += en - b;
   wsync();
@@ -388,17 +412,17 @@ __device__ void dec_neighbour(W& w, u32 x, const Clo& c) {
   u8 s = ST_R;
   if (w.bm) {
     if (xbit(w, x)) return;
-    if (c.on) s = w.st[x];
+    if (c.on) s = w.vs[x].st;
   } else {
-    s = w.st[x];
+    s = w.vs[x].st;
     if (s == ST_X) return;
   }
-  u32 old = atomicSub(&w.dR[x], 1u);
+  u32 old = atomicSub(&w.vs[x].dR, 1u);
   if (c.on && s == ST_R) {
-    u32 dc = w.dC[x];
+    u32 dc = w.vs[x].dC;
     if (dc <= c.kl && old + dc == c.kl + 1) {
-      u32* wp = (u32*)((uintptr_t)(w.queued + x) & ~(uintptr_t)3);
-      u32 sh = ((uintptr_t)(w.queued + x) & 3) * 8;
+      u32* wp = (u32*)((uintptr_t)(&w.vs[x].queued) & ~(uintptr_t)3);
+      u32 sh = ((uintptr_t)(&w.vs[x].queued) & 3) * 8;
       u32 o2 = atomicOr(wp, 1u << sh);
       if (!((o2 >> sh) & 0xffu)) c.next[atomicAdd(c.cnt, 1u)] = x;
     }
@@ -444,13 +468,19 @@ __device__ __forceinline__ void remove_batch(W& w, const u32* list, u32 m, const
   }
   wsync();
   // phase 2: mark X, histogram, trail, queued flags
-  for (u32 i = LANE; i < m; i += 64) {
-    u32 v = list[i];
-    w.st[v] = ST_X;
-    if (w.bm) atomicOr(&w.xb[v >> 5], 1u << (v & 31));
-    w.queued[v] = 0;
-    atomicSub(&hist[w.dC[v]], 1u);
-    w.trail[w.ntrail + i] = v;
+  for (u32 i0 = 0; i0 < m; i0 += 64) {
+    const u32 i = i0 + LANE;
+    const bool on = i < m;
+    u32 d = 0;
+    if (on) {
+      const u32 v = list[i];
+      d = w.vs[v].dC;
+      w.vs[v].st = ST_X;
+      if (w.bm) atomicOr(&w.xb[v >> 5], 1u << (v & 31));
+      w.vs[v].queued = 0;
+      w.trail[w.ntrail + i] = v;
+    }
+    hist_add(hist, on, d, -1);
   }
   wsync();
   // phase 3: fill the holes the batch leaves in R's list with the surviving
@@ -462,7 +492,7 @@ __device__ __forceinline__ void remove_batch(W& w, const u32* list, u32 m, const
   }
   for (u32 t = tail + LANE; t < w.nR; t += 64) {
     u32 x = w.rlist[t];
-    if (w.st[x] == ST_R) w.front[atomicAdd(&w.L->nk, 1u)] = x;
+    if (w.vs[x].st == ST_R) w.front[atomicAdd(&w.L->nk, 1u)] = x;
   }
   wsync();
   const u32 nh = bcast(w.L->nh);
@@ -481,14 +511,14 @@ __device__ __forceinline__ void remove_batch(W& w, const u32* list, u32 m, const
     const u32 nR = w.nR;
     scan_adj<u32>(
         w, live_after, [&](u32 i) { return i < nR ? w.rlist[i] : w.clist[i - nR]; },
-        [&](u32 x) { return (u32)(w.st[x] == ST_R); },
+        [&](u32 x) { return (u32)(w.vs[x].st == ST_R); },
         [&](u32 i, u32 v, u32 r) {
           S[2 * i] = v;
-          S[2 * i + 1] = w.dR[v];
-          w.dR[v] = r;
-          if (c.on && i < nR && w.dC[v] + r <= c.kl) {
-            u32* wp = (u32*)((uintptr_t)(w.queued + v) & ~(uintptr_t)3);
-            u32 sh = ((uintptr_t)(w.queued + v) & 3) * 8;
+          S[2 * i + 1] = w.vs[v].dR;
+          w.vs[v].dR = r;
+          if (c.on && i < nR && w.vs[v].dC + r <= c.kl) {
+            u32* wp = (u32*)((uintptr_t)(&w.vs[v].queued) & ~(uintptr_t)3);
+            u32 sh = ((uintptr_t)(&w.vs[v].queued) & 3) * 8;
             u32 o2 = atomicOr(wp, 1u << sh);
             if (!((o2 >> sh) & 0xffu)) c.next[atomicAdd(c.cnt, 1u)] = v;
           }
@@ -506,17 +536,23 @@ __device__ void undo_batch(W& w, u32 start, u32 m, bool recount) {
     // restore the recounted dR values first (same live set as at the batch)
     const u32 nl = w.sav[w.nsav - 1];
     const u32* S = w.sav + w.nsav - 1 - 2 * nl;
-    for (u32 i = LANE; i < nl; i += 64) w.dR[S[2 * i]] = S[2 * i + 1];
+    for (u32 i = LANE; i < nl; i += 64) w.vs[S[2 * i]].dR = S[2 * i + 1];
     w.nsav -= 2 * nl + 1;
     wsync();
   }
-  for (u32 i = LANE; i < m; i += 64) {
-    u32 v = E[i];
-    w.st[v] = ST_R;
-    if (w.bm) atomicAnd(&w.xb[v >> 5], ~(1u << (v & 31)));
-    atomicAdd(&hist[w.dC[v]], 1u);
-    w.rlist[w.nR + i] = v;
-    w.pos[v] = w.nR + i;
+  for (u32 i0 = 0; i0 < m; i0 += 64) {
+    const u32 i = i0 + LANE;
+    const bool on = i < m;
+    u32 d = 0;
+    if (on) {
+      const u32 v = E[i];
+      w.vs[v].st = ST_R;
+      if (w.bm) atomicAnd(&w.xb[v >> 5], ~(1u << (v & 31)));
+      d = w.vs[v].dC;
+      w.rlist[w.nR + i] = v;
+      w.pos[v] = w.nR + i;
+    }
+    hist_add(hist, on, d, 1);
   }
   w.nR += m;
   wsync();
@@ -525,7 +561,7 @@ __device__ void undo_batch(W& w, u32 start, u32 m, bool recount) {
     scan_adj<u32>(
         w, m, [&](u32 i) { return E[i]; },
         [&](u32 x) {
-          if (w.bm ? !xbit(w, x) : w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
+          if (w.bm ? !xbit(w, x) : w.vs[x].st != ST_X) atomicAdd(&w.vs[x].dR, 1u);
           return 0u;
         },
         [&](u32, u32, u32) {});
@@ -535,7 +571,7 @@ __device__ void undo_batch(W& w, u32 start, u32 m, bool recount) {
       const u32 b = w.off[v], e = w.off[v + 1];
       for (u32 j = b + LANE; j < e; j += 64) {
         u32 x = w.adj[j];
-        if (w.bm ? !xbit(w, x) : w.st[x] != ST_X) atomicAdd(&w.dR[x], 1u);
+        if (w.bm ? !xbit(w, x) : w.vs[x].st != ST_X) atomicAdd(&w.vs[x].dR, 1u);
       }
       w.edges += e - b;
     }
@@ -606,7 +642,7 @@ __device__ u32 cur_kl(const W& w) { return aload(&w.Q->kl); }
 // Node processing.
 // ---------------------------------------------------------------------------
 __device__ bool cond12(const W& w, u32 v, u32 Cs, u32 ubD, u32 kl) {
-  u32 dc = w.dC[v], dr = w.dR[v];
+  u32 dc = w.vs[v].dC, dr = w.vs[v].dR;
   if ((w.flags & QF_RED1) && min(dr + dc, dc + w.h - Cs - 1) <= kl) return true;
   if ((w.flags & QF_RED2) && ubD < w.dist[v]) return true;
   return false;
@@ -614,9 +650,9 @@ __device__ bool cond12(const W& w, u32 v, u32 Cs, u32 ubD, u32 kl) {
 
 __device__ void try_push(W& w, u32 x, u32 Cs, u32 ubD, u32 kl, u32* list,
                                          u32* cnt) {
-  if (w.st[x] == ST_R && !w.queued[x] && cond12(w, x, Cs, ubD, kl)) {
-    u32* wp = (u32*)((uintptr_t)(w.queued + x) & ~(uintptr_t)3);
-    u32 sh = ((uintptr_t)(w.queued + x) & 3) * 8;
+  if (w.vs[x].st == ST_R && !w.vs[x].queued && cond12(w, x, Cs, ubD, kl)) {
+    u32* wp = (u32*)((uintptr_t)(&w.vs[x].queued) & ~(uintptr_t)3);
+    u32 sh = ((uintptr_t)(&w.vs[x].queued) & 3) * 8;
     u32 old = atomicOr(wp, 1u << sh);
     if (!((old >> sh) & 0xffu)) list[atomicAdd(cnt, 1u)] = x;
   }
@@ -679,7 +715,7 @@ __device__ bool tight_exists(const W& w, u32 kl) {
   u32 f = 0;
   for (u32 i = LANE; i < w.nC; i += 64) {
     u32 u = w.clist[i];
-    if (w.dC[u] + w.dR[u] == kl + 1 && w.dR[u] > 0) f = 1;
+    if (w.vs[u].dC + w.vs[u].dR == kl + 1 && w.vs[u].dR > 0) f = 1;
   }
   return __ballot(f) != 0;
 }
@@ -695,7 +731,7 @@ __device__ u32 unr_bound(W& w, u32 k) {
   const u32 top = h + 1;
   for (u32 i = LANE; i <= top; i += 64) c[i] = 0;
   wsync();
-  for (u32 i = LANE; i < w.nC; i += 64) atomicAdd(&c[min((u32)w.dC[w.clist[i]], top)], 1u);
+  for (u32 i = LANE; i < w.nC; i += 64) atomicAdd(&c[min((u32)w.vs[w.clist[i]].dC, top)], 1u);
   wsync();
   u32 res = INV;
   if (LANE == 0) {
@@ -740,8 +776,8 @@ struct NodeOut {
 
 __device__ __forceinline__ MlpFeat cand_features(const W& w, u32 x, u32 Cs, u32 kl) {
   MlpFeat f;
-  f.dC = w.dC[x];
-  f.dR = w.dR[x];
+  f.dC = w.vs[x].dC;
+  f.dR = w.vs[x].dR;
   f.dist = w.dist[x];
   f.core = w.core[x];
   f.slack = (int)(f.dC + min(f.dR, w.h - Cs - 1)) - (int)kl;
@@ -827,7 +863,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     const u32 Cs = w.nC;
     if (Cs >= w.l && Cs <= h) {
       u32 mn = INV;
-      for (u32 i = LANE; i < Cs; i += 64) mn = min(mn, (u32)w.dC[w.clist[i]]);
+      for (u32 i = LANE; i < Cs; i += 64) mn = min(mn, (u32)w.vs[w.clist[i]].dC);
       mn = wmin(mn);
       if (mn != INV && mn >= kl) offer(w, mn);
       u32 nk = cur_kl(w);
@@ -844,7 +880,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
       wsync();
       for (u32 i = LANE; i < Cs; i += 64) {
         u32 u = w.clist[i];
-        if (w.dC[u] + w.dR[u] == kl + 1 && w.dR[u] > 0) w.work[atomicAdd(&L->cnt, 1u)] = u;
+        if (w.vs[u].dC + w.vs[u].dR == kl + 1 && w.vs[u].dR > 0) w.work[atomicAdd(&L->cnt, 1u)] = u;
       }
       wsync();
       const u32 nt = bcast(L->cnt);
@@ -854,7 +890,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
         const u32 b = w.off[u], e = w.off[u + 1];
         for (u32 j = b; j < e; j++) {
           u32 x = w.adj[j];
-          if (w.st[x] == ST_R) {
+          if (w.vs[x].st == ST_R) {
             move_c(w, x);
             if (w.nC > h) {
               over = true;
@@ -885,7 +921,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     u32 b = INV;
     for (u32 i = LANE; i < Cs; i += 64) {
       u32 u = w.clist[i];
-      u32 dc = w.dC[u], dr = w.dR[u];
+      u32 dc = w.vs[u].dC, dr = w.vs[u].dR;
       b = min(b, min(dc + h - Cs, dc + dr));
     }
     ub = wmin(b);
@@ -896,7 +932,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   ph_switch(L, tp, ph, PH_SCORE);
   if (LANE == 0) L->nfront = 0;
   wsync();
-  const bool single0 = (Cs == 1 && w.dC[w.clist[0]] == 0);
+  const bool single0 = (Cs == 1 && w.vs[w.clist[0]].dC == 0);
   if (w.bm) {
     // tb = C, so "x is R" is two LDS bit tests
     for (u32 i = LANE; i < Cs; i += 64) tset(w, w.clist[i]);
@@ -904,13 +940,29 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   }
   for (u32 i = 0; i < Cs; i++) {
     u32 u = w.clist[i];
-    u64 r = recip32(w.dC[u]);
+    u64 r = recip32(w.vs[u].dC);
     const u32 b = w.off[u], e = w.off[u + 1];
-    for (u32 j = b + LANE; j < e; j += 64) {
-      u32 x = w.adj[j];
-      if (w.bm ? (xbit(w, x) || tbit(w, x)) : w.st[x] != ST_R) continue;
-      u64 old = atomicAdd((unsigned long long*)&w.acc[x], (unsigned long long)r);
-      if (old == 0) w.front[atomicAdd(&L->nfront, 1u)] = x;
+    // four 64-entry chunks per step: loads, tests and atomics of the four
+    // are issued before any returned value is used (the learned variant
+    // keeps one chunk per step: it has no registers to spare)
+    constexpr int U = LEARN ? 1 : 4;
+    for (u32 j = b + LANE; j < e; j += 64 * U) {
+      u32 x[U];
+      bool ok[U];
+#pragma unroll
+      for (int k = 0; k < U; k++) x[k] = j + 64 * k < e ? w.adj[j + 64 * k] : 0u;
+#pragma unroll
+      for (int k = 0; k < U; k++)
+        ok[k] = j + 64 * k < e &&
+                (w.bm ? !(xbit(w, x[k]) || tbit(w, x[k])) : w.vs[x[k]].st == ST_R);
+      u64 old[U];
+#pragma unroll
+      for (int k = 0; k < U; k++)
+        old[k] = ok[k] ? atomicAdd((unsigned long long*)&w.acc[x[k]], (unsigned long long)r)
+                       : 1ull;
+#pragma unroll
+      for (int k = 0; k < U; k++)
+        if (old[k] == 0) w.front[atomicAdd(&L->nfront, 1u)] = x[k];
     }
     w.edges += e - b;
   }
@@ -937,7 +989,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
       const u32 x0 = w.front[i], x1 = w.front[i + 64], x2 = w.front[i + 128],
                 x3 = w.front[i + 192];
       const u64 a0 = w.acc[x0], a1 = w.acc[x1], a2 = w.acc[x2], a3 = w.acc[x3];
-      const u32 r0 = w.dR[x0], r1 = w.dR[x1], r2 = w.dR[x2], r3 = w.dR[x3];
+      const u32 r0 = w.vs[x0].dR, r1 = w.vs[x1].dR, r2 = w.vs[x2].dR, r3 = w.vs[x3].dR;
       consider(x0, a0 + rterm(r0, w.dmax));
       consider(x1, a1 + rterm(r1, w.dmax));
       consider(x2, a2 + rterm(r2, w.dmax));
@@ -945,7 +997,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     }
     for (; i < nf; i += 64) {
       const u32 x = w.front[i];
-      consider(x, w.acc[x] + rterm(w.dR[x], w.dmax));
+      consider(x, w.acc[x] + rterm(w.vs[x].dR, w.dmax));
     }
   }
   const u64 gbest = wmax64(best);
@@ -964,7 +1016,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   ph_switch(L, tp, ph, PH_DOMS);
   u32 nD = 0;
   if ((w.flags & QF_DOMS) && w.limit_doms > 0) {
-    const u32 lu = w.dC[us] + w.dR[us];
+    const u32 lu = w.vs[us].dC + w.vs[us].dR;
     const u32 ub0 = w.off[us], ue = w.off[us + 1];
     const bool bm = w.bm;
     if (bm) {
@@ -981,7 +1033,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     u32 ed = 0, ntest = 0;
     for (u32 i = LANE; i < nf; i += 64) {
       u32 x = w.front[i];
-      if (x == us || w.dC[x] + w.dR[x] > lu + 1) continue;
+      if (x == us || w.vs[x].dC + w.vs[x].dR > lu + 1) continue;
       ntest++;
       bool dom = true;
       const u32 b = w.off[x], e = w.off[x + 1];
@@ -1002,7 +1054,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
         for (u32 j = b; j < e; j++) {
           u32 y = w.adj[j];
           ed++;
-          if (w.st[y] == ST_X) continue;
+          if (w.vs[y].st == ST_X) continue;
           if (y != us && !w.nmark[y]) {
             dom = false;
             break;
@@ -1035,7 +1087,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
       for (u32 i = LANE; i < nc; i += 64) {
         u32 x = w.work[i];
         if (x == INV) continue;
-        u64 s = (single0 ? w.acc[x] - (1ull << 32) : w.acc[x]) + rterm(w.dR[x], w.dmax);
+        u64 s = (single0 ? w.acc[x] - (1ull << 32) : w.acc[x]) + rterm(w.vs[x].dR, w.dmax);
         if (bx == INV || s > bs || (s == bs && x < bx)) {
           bs = s;
           bx = x;
@@ -1113,12 +1165,12 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
   const u8 fill = form ? ST_X : ST_R;
   // queued[], nmark[] and acc[] are all-zero between nodes (every node
   // clears what it set), so only the status needs initialising
-  for (u32 v = LANE; v < K; v += 64) w.st[v] = fill;
+  for (u32 v = LANE; v < K; v += 64) w.vs[v].st = fill;
   wsync();
-  for (u32 i = LANE; i < nL; i += 64) w.st[Lst[i]] = form ? ST_R : ST_X;
+  for (u32 i = LANE; i < nL; i += 64) w.vs[Lst[i]].st = form ? ST_R : ST_X;
   wsync();
   for (u32 i = LANE; i < nC; i += 64) {
-    w.st[C[i]] = ST_C;
+    w.vs[C[i]].st = ST_C;
     w.clist[i] = C[i];
   }
   for (u32 i = LANE; i < SCS_HCAP + 4; i += 64) w.L->hist[i] = 0;
@@ -1128,7 +1180,7 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
   u32 base = 0;
   for (u32 v0 = 0; v0 < K; v0 += 64) {
     u32 v = v0 + LANE;
-    bool r = v < K && w.st[v] == ST_R;
+    bool r = v < K && w.vs[v].st == ST_R;
     u64 bm = __ballot(r);
     if (r) {
       u32 p = base + __popcll(bm & lanes_below());
@@ -1147,22 +1199,22 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
     // take away.  dR(v) = deg(v) - |N(v) & X| - |N(v) & C|, dC(v) = |N(v) & C|;
     // counters of X vertices are never read, so their values do not matter.
     for (u32 v = LANE; v < K; v += 64) {
-      w.dC[v] = 0;
-      w.dR[v] = w.off[v + 1] - w.off[v];
+      w.vs[v].dC = 0;
+      w.vs[v].dR = w.off[v + 1] - w.off[v];
     }
     wsync();
     if (nL >= 8) {
       scan_adj<u32>(
           w, nL, [&](u32 i) { return Lst[i]; },
           [&](u32 y) {
-            atomicSub(&w.dR[y], 1u);
+            atomicSub(&w.vs[y].dR, 1u);
             return 0u;
           },
           [&](u32, u32, u32) {});
     } else {
       for (u32 i = 0; i < nL; i++) {
         const u32 x = Lst[i], b = w.off[x], e = w.off[x + 1];
-        for (u32 j = b + LANE; j < e; j += 64) atomicSub(&w.dR[w.adj[j]], 1u);
+        for (u32 j = b + LANE; j < e; j += 64) atomicSub(&w.vs[w.adj[j]].dR, 1u);
         w.edges += e - b;
       }
     }
@@ -1172,15 +1224,17 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
       const u32 c = C[i], b = w.off[c], e = w.off[c + 1];
       for (u32 j = b + LANE; j < e; j += 64) {
         const u32 y = w.adj[j];
-        w.dC[y] = (u16)(w.dC[y] + 1);
-        w.dR[y] -= 1;
+        w.vs[y].dC = (u16)(w.vs[y].dC + 1);
+        w.vs[y].dR -= 1;
       }
       w.edges += e - b;
       wsync();
     }
-    for (u32 i = LANE; i < nR; i += 64) {
-      const u32 v = w.rlist[i];
-      atomicAdd(&w.L->hist[min((u32)w.dC[v], w.h + 1)], 1u);
+    for (u32 i0 = 0; i0 < nR; i0 += 64) {
+      const u32 i = i0 + LANE;
+      const bool on = i < nR;
+      const u32 d = on ? min((u32)w.vs[w.rlist[i]].dC, w.h + 1) : 0u;
+      hist_add(w.L->hist, on, d, 1);
     }
     w.ntrail = 0;
     w.nframes = 0;
@@ -1193,13 +1247,13 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
   scan_adj<u64>(
       w, nR + nC, [&](u32 i) { return i < nR ? w.rlist[i] : w.clist[i - nR]; },
       [&](u32 x) {
-        const u8 s = w.st[x];
+        const u8 s = w.vs[x].st;
         return ((u64)(s == ST_C) << 32) | (u64)(s == ST_R);
       },
       [&](u32 i, u32 v, u64 cr) {
         const u32 c = (u32)(cr >> 32), r = (u32)cr;
-        w.dC[v] = (u16)c;
-        w.dR[v] = r;
+        w.vs[v].dC = (u16)c;
+        w.vs[v].dR = r;
         if (i < nR) atomicAdd(&w.L->hist[min(c, w.h + 1)], 1u);
       });
   w.ntrail = 0;
@@ -1232,7 +1286,7 @@ __device__ __forceinline__ u32 serialize_z(W& w, u32 fi, u32* dst, u32 qslot) {
     if (v < w.K) {
       // at the frame: nmark 2 = ustar or D (out in Z), 1 = changed after the
       // frame's mark (was R), 0 = unchanged (current status holds)
-      u8 s = w.st[v], m = w.nmark[v];
+      u8 s = w.vs[v].st, m = w.nmark[v];
       bool inRz = (m == 1) || (m == 0 && s == ST_R);
       bool inC = (m == 0 && s == ST_C);
       take = form ? inRz : (!inRz && !inC);

@@ -90,10 +90,18 @@ struct WaveHdr {
   u32 pad[4];
 };
 
+// Search state of one vertex, packed so that the fields a rule or counter
+// update reads together come from one cache line.
+struct VState {
+  u32 dR;     // |N(v) & R|
+  u16 dC;     // |N(v) & C|
+  u8 st;      // ST_R / ST_C / ST_X
+  u8 queued;  // queued for removal in the current closure round
+};
+
 // Offsets (bytes) of each per-wave array inside one wave's region.
 struct WaveLayout {
-  u64 st, nmark, queued, dC, dR, pos, rlist, acc, work, work2, front, aux, trail, sav, frames,
-      clist, hist, hdr;
+  u64 vs, nmark, pos, rlist, acc, work, work2, front, aux, trail, sav, frames, clist, hist, hdr;
   u64 stride;
 };
 
