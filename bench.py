@@ -175,6 +175,7 @@ def main():
                  "algorithmic_bytes": int(s1["prep_bytes"] - s0["prep_bytes"])},
         "search_phases": phase_shares(s0, s1),
         "domination": {k: int(s1[k] - s0[k]) for k in ("dom_front", "dom_cand", "dom_found")},
+        "rule12_full_rescans": {k: int(s1[k] - s0[k]) for k in ("full_scans", "full_rows")},
         "heuristic_host_us": round(s1["heu_us"] - s0["heu_us"], 1),
         "core_decomposition_us": round(s1["core_us"], 1),
     }

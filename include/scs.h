@@ -169,6 +169,8 @@ typedef struct scs_stats {
   uint64_t dom_front;      /* frontier vertices summed over branching nodes */
   uint64_t dom_cand;       /* frontier vertices tested for domination */
   uint64_t dom_found;      /* tested vertices found dominated by ustar */
+  uint64_t full_scans;     /* rule 1/2 closures that rescanned all of R */
+  uint64_t full_rows;      /* R vertices visited by those rescans */
 } scs_stats;
 int scs_engine_stats(const scs_engine* e, scs_stats* s);
 

@@ -480,7 +480,10 @@ static inline u64 rterm(u32 dR, u32 dmax) { return dmax ? (((u64)dR << 32) / dma
 //   B_i = (C+u*+d_i,  R-u*-d_0..d_{i-1})   i = 0..|D|-1
 //   Z   = (C,         R-u*-D)
 // Rules 3 and 1/2 run to a fixpoint inside a node; domination is tested
-// against the neighbours still in C or R; the search stops as soon as
+// against the neighbours still in C or R, for the frontier vertices (R with
+// dC > 0) adjacent to ustar: a sound subset of FindDoms' candidates (any
+// vertex whose live neighbours all lie in N[ustar] may be dropped from the
+// "exclude ustar" branch); the search stops as soon as
 // kl == ku = min(core[q], h-1), which no solution can exceed.
 // ---------------------------------------------------------------------------
 struct Params {
@@ -650,7 +653,8 @@ struct RecomputeSolver {
     if (P.use_doms && P.limit_doms > 0) {
       for (u32 j = L.off[u]; j < L.off[u + 1]; j++) mark[L.adj[j]] = 1;
       for (u32 v : R) {
-        if (v == u || dC[v] == 0) continue;
+        // candidates: frontier vertices adjacent to ustar
+        if (v == u || dC[v] == 0 || !mark[v]) continue;
         bool dom = true;
         u64 s = 0;
         for (u32 j = L.off[v]; j < L.off[v + 1]; j++) {
@@ -934,7 +938,8 @@ struct TrailSolver {
       const u32 lu = dC[ustar] + dR[ustar];
       for (u32 j = L.off[ustar]; j < L.off[ustar + 1]; j++) nmark[L.adj[j]] = 1;
       for (u32 v : front) {
-        if (v == ustar || dC[v] + dR[v] > lu + 1) continue;
+        // candidates: frontier vertices adjacent to ustar
+        if (v == ustar || !nmark[v] || dC[v] + dR[v] > lu + 1) continue;
         bool dom = true;
         for (u32 j = L.off[v]; j < L.off[v + 1]; j++) {
           u32 w = L.adj[j];

@@ -361,7 +361,6 @@ static WaveLayout make_layout(u32 kcap) {
   L.nmark = o; o = align256(o + k4);
   L.pos = o; o = align256(o + 4 * k4);
   L.rlist = o; o = align256(o + 4 * k4);
-  L.acc = o; o = align256(o + 8 * k4);
   L.work = o; o = align256(o + 4 * (k4 + 64));
   L.work2 = o; o = align256(o + 4 * (k4 + 64));
   L.front = o; o = align256(o + 4 * (k4 + 64));
@@ -1042,6 +1041,8 @@ extern "C" int scs_engine_stats(const scs_engine* E, scs_stats* s) {
   s->dom_front = gl.dom_front;
   s->dom_cand = gl.dom_cand;
   s->dom_found = gl.dom_found;
+  s->full_scans = gl.full_scans;
+  s->full_rows = gl.full_rows;
   s->core_us = E->g->core_us;
   unsigned long long pb = 0;
   HIPCHK(hipMemcpy(&pb, E->d_prepbytes, sizeof(pb), hipMemcpyDeviceToHost));

@@ -131,7 +131,6 @@ struct W {
   u8* nmark;
   u32* pos;
   u32* rlist;
-  u64* acc;
   u32* work;
   u32* work2;
   u32* front;
@@ -174,7 +173,6 @@ __device__ void bind(W& w, const EngineArgs& A, u32 wave, Lds* L) {
   w.nmark = base + A.lay.nmark;
   w.pos = (u32*)(base + A.lay.pos);
   w.rlist = (u32*)(base + A.lay.rlist);
-  w.acc = (u64*)(base + A.lay.acc);
   w.work = (u32*)(base + A.lay.work);
   w.work2 = (u32*)(base + A.lay.work2);
   w.front = (u32*)(base + A.lay.front);
@@ -321,19 +319,22 @@ __device__ __forceinline__ void hist_add(u32* hist, bool on, u32 d, int delta) {
 // marks the batch X; its undo marks the batch R first and then increments,
 // so batch members see each other both ways.
 // ---------------------------------------------------------------------------
+// U: adjacency chunks in flight per step (the learned kernel variant uses 1:
+// it has no registers to spare).
+template <int U = 4>
 __device__ void move_c(W& w, u32 v) {
   const u32 b = w.off[v], e = w.off[v + 1];
   u32* hist = w.L->hist;
-  for (u32 j = b + LANE; j < e; j += 256) {
-    u32 x[4];
-    VState vs[4];
+  for (u32 j = b + LANE; j < e; j += 64 * U) {
+    u32 x[U];
+    VState vs[U];
 #pragma unroll
-    for (int k = 0; k < 4; k++) x[k] = j + 64 * k < e ? w.adj[j + 64 * k] : INV;
+    for (int k = 0; k < U; k++) x[k] = j + 64 * k < e ? w.adj[j + 64 * k] : INV;
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < U; k++)
       if (x[k] != INV) vs[k] = w.vs[x[k]];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < U; k++) {
       const bool live = x[k] != INV && vs[k].st != ST_X;
       const u32 dc = live ? vs[k].dC : 0u;
       const bool r = live && vs[k].st == ST_R;
@@ -363,6 +364,7 @@ __device__ void move_c(W& w, u32 v) {
   wsync();
 }
 
+template <int U = 4>
 __device__ void undo_move(W& w, u32 v) {
   u32* hist = w.L->hist;
   if (LANE == 0) {
@@ -375,16 +377,16 @@ __device__ void undo_move(W& w, u32 v) {
   w.nC--;
   wsync();
   const u32 b = w.off[v], en = w.off[v + 1];
-  for (u32 j = b + LANE; j < en; j += 256) {
-    u32 x[4];
-    VState vs[4];
+  for (u32 j = b + LANE; j < en; j += 64 * U) {
+    u32 x[U];
+    VState vs[U];
 #pragma unroll
-    for (int k = 0; k < 4; k++) x[k] = j + 64 * k < en ? w.adj[j + 64 * k] : INV;
+    for (int k = 0; k < U; k++) x[k] = j + 64 * k < en ? w.adj[j + 64 * k] : INV;
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < U; k++)
       if (x[k] != INV) vs[k] = w.vs[x[k]];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < U; k++) {
       const bool live = x[k] != INV && vs[k].st != ST_X;
       const u32 dc = live ? vs[k].dC : 1u;
       const bool r = live && vs[k].st == ST_R;
@@ -579,12 +581,13 @@ __device__ void undo_batch(W& w, u32 start, u32 m, bool recount) {
   wsync();
 }
 
+template <int U = 4>
 __device__ void undo_to(W& w, u32 mark) {
   while (w.ntrail > mark) {
     const u32 e = w.trail[w.ntrail - 1];
     if (e & TR_MOVE) {
       w.ntrail--;
-      undo_move(w, e & TR_VMASK);
+      undo_move<U>(w, e & TR_VMASK);
     } else {
       const u32 m = e & TR_MMASK;  // batch marker
       const u32 start = w.ntrail - 1 - m;
@@ -679,14 +682,34 @@ __device__ __forceinline__ bool r12(W& w, u32 from, bool full, u32 kl) {
     }
   }
   if (full) {
+    if (LANE == 0) {
+      atomicAdd((unsigned long long*)&w.gl->full_scans, 1ull);
+      atomicAdd((unsigned long long*)&w.gl->full_rows, (unsigned long long)w.nR);
+    }
     for (u32 i = LANE; i < w.nR; i += 64) try_push(w, w.rlist[i], Cs, ubD, kl, w.work, &L->cnt);
   } else {
-    for (u32 t = from; t < w.ntrail; t++) {
-      const u32 e = w.trail[t];
-      if (e & (TR_MOVE | TR_MARK)) continue;
-      const u32 b = w.off[e], en = w.off[e + 1];
-      for (u32 j = b + LANE; j < en; j += 64) try_push(w, w.adj[j], Cs, ubD, kl, w.work, &L->cnt);
-      w.edges += en - b;
+    // vertices removed since `from` (trail entries without the move or
+    // batch-marker flags), compacted into work2, then their neighbours
+    // checked with the load-balanced scan
+    u32 nrm = 0;
+    for (u32 t0 = from; t0 < w.ntrail; t0 += 64) {
+      const u32 t = t0 + LANE;
+      const u32 e = t < w.ntrail ? w.trail[t] : TR_MARK;
+      const bool plain = !(e & (TR_MOVE | TR_MARK));
+      const u64 bm = __ballot(plain);
+      if (plain) w.work2[nrm + __popcll(bm & lanes_below())] = e;
+      nrm += __popcll(bm);
+    }
+    wsync();
+    if (nrm) {
+      const u32* rm = w.work2;
+      scan_adj<u32>(
+          w, nrm, [&](u32 i) { return rm[i]; },
+          [&](u32 x) {
+            try_push(w, x, Cs, ubD, kl, w.work, &L->cnt);
+            return 0u;
+          },
+          [&](u32, u32, u32) {});
     }
   }
   wsync();
@@ -781,7 +804,7 @@ __device__ __forceinline__ MlpFeat cand_features(const W& w, u32 x, u32 Cs, u32 
   f.dist = w.dist[x];
   f.core = w.core[x];
   f.slack = (int)(f.dC + min(f.dR, w.h - Cs - 1)) - (int)kl;
-  f.cscore = (float)((double)(w.acc[x] + rterm(f.dR, w.dmax)) * (1.0 / 4294967296.0));
+  f.cscore = (float)((double)(w.vs[x].acc + rterm(f.dR, w.dmax)) * (1.0 / 4294967296.0));
   return f;
 }
 
@@ -838,12 +861,12 @@ __device__ void clear_acc(W& w, u32 nf) {
   u32 i = LANE;
   for (; i + 192 < nf; i += 256) {
     const u32 x0 = w.front[i], x1 = w.front[i + 64], x2 = w.front[i + 128], x3 = w.front[i + 192];
-    w.acc[x0] = 0;
-    w.acc[x1] = 0;
-    w.acc[x2] = 0;
-    w.acc[x3] = 0;
+    w.vs[x0].acc = 0;
+    w.vs[x1].acc = 0;
+    w.vs[x2].acc = 0;
+    w.vs[x3].acc = 0;
   }
-  for (; i < nf; i += 64) w.acc[w.front[i]] = 0;
+  for (; i < nf; i += 64) w.vs[w.front[i]].acc = 0;
   wsync();
 }
 
@@ -891,7 +914,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
         for (u32 j = b; j < e; j++) {
           u32 x = w.adj[j];
           if (w.vs[x].st == ST_R) {
-            move_c(w, x);
+            move_c<LEARN ? 1 : 4>(w, x);
             if (w.nC > h) {
               over = true;
               break;
@@ -958,7 +981,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
       u64 old[U];
 #pragma unroll
       for (int k = 0; k < U; k++)
-        old[k] = ok[k] ? atomicAdd((unsigned long long*)&w.acc[x[k]], (unsigned long long)r)
+        old[k] = ok[k] ? atomicAdd((unsigned long long*)&w.vs[x[k]].acc, (unsigned long long)r)
                        : 1ull;
 #pragma unroll
       for (int k = 0; k < U; k++)
@@ -988,16 +1011,15 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     for (; !LEARN && i + 192 < nf; i += 256) {
       const u32 x0 = w.front[i], x1 = w.front[i + 64], x2 = w.front[i + 128],
                 x3 = w.front[i + 192];
-      const u64 a0 = w.acc[x0], a1 = w.acc[x1], a2 = w.acc[x2], a3 = w.acc[x3];
-      const u32 r0 = w.vs[x0].dR, r1 = w.vs[x1].dR, r2 = w.vs[x2].dR, r3 = w.vs[x3].dR;
-      consider(x0, a0 + rterm(r0, w.dmax));
-      consider(x1, a1 + rterm(r1, w.dmax));
-      consider(x2, a2 + rterm(r2, w.dmax));
-      consider(x3, a3 + rterm(r3, w.dmax));
+      const VState v0 = w.vs[x0], v1 = w.vs[x1], v2 = w.vs[x2], v3 = w.vs[x3];
+      consider(x0, v0.acc + rterm(v0.dR, w.dmax));
+      consider(x1, v1.acc + rterm(v1.dR, w.dmax));
+      consider(x2, v2.acc + rterm(v2.dR, w.dmax));
+      consider(x3, v3.acc + rterm(v3.dR, w.dmax));
     }
     for (; i < nf; i += 64) {
       const u32 x = w.front[i];
-      consider(x, w.acc[x] + rterm(w.vs[x].dR, w.dmax));
+      consider(x, w.vs[x].acc + rterm(w.vs[x].dR, w.dmax));
     }
   }
   const u64 gbest = wmax64(best);
@@ -1028,12 +1050,15 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     }
     if (LANE == 0) L->ncand = 0;
     wsync();
-    // x is dominated when every live neighbour is us or a neighbour of us;
-    // one candidate per lane: the scans mostly stop after a few entries
+    // candidates: frontier vertices (R with dC > 0) adjacent to us, a sound
+    // subset of the dominated set; x is dominated when every live neighbour
+    // is us or a neighbour of us.  One candidate per lane: the scans mostly
+    // stop after a few entries.
     u32 ed = 0, ntest = 0;
-    for (u32 i = LANE; i < nf; i += 64) {
-      u32 x = w.front[i];
-      if (x == us || w.vs[x].dC + w.vs[x].dR > lu + 1) continue;
+    for (u32 jj = ub0 + LANE; jj < ue; jj += 64) {
+      const u32 x = w.adj[jj];
+      const VState xs = w.vs[x];
+      if (xs.st != ST_R || xs.dC == 0 || xs.dC + xs.dR > lu + 1) continue;
       ntest++;
       bool dom = true;
       const u32 b = w.off[x], e = w.off[x + 1];
@@ -1087,7 +1112,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
       for (u32 i = LANE; i < nc; i += 64) {
         u32 x = w.work[i];
         if (x == INV) continue;
-        u64 s = (single0 ? w.acc[x] - (1ull << 32) : w.acc[x]) + rterm(w.vs[x].dR, w.dmax);
+        u64 s = (single0 ? w.vs[x].acc - (1ull << 32) : w.vs[x].acc) + rterm(w.vs[x].dR, w.dmax);
         if (bx == INV || s > bs || (s == bs && x < bx)) {
           bs = s;
           bx = x;
@@ -1490,13 +1515,13 @@ __global__ void __launch_bounds__(64, 4) __attribute__((flatten)) scs_epoch(Engi
       kl_seen = o.klc;
       const Clo none{0, 0, nullptr, nullptr};
       remove_batch(w, f->D, o.nD, none);
-      move_c(w, o.ustar);
+      move_c<LEARN ? 1 : 4>(w, o.ustar);
     } else {
       // backtrack to the next pending child
       bool cont = false;
       while (w.nframes > 0) {
         Frame* f = &w.frames[w.nframes - 1];
-        undo_to(w, f->mark);
+        undo_to<LEARN ? 1 : 4>(w, f->mark);
         from = f->mark;
         kl_seen = f->kl;
         const u32 next = f->next + 1;
@@ -1507,8 +1532,8 @@ __global__ void __launch_bounds__(64, 4) __attribute__((flatten)) scs_epoch(Engi
         if (next <= nD) {
           const u32 i = next - 1;
           remove_batch(w, f->D, i, none);
-          move_c(w, f->ustar);
-          move_c(w, f->D[i]);
+          move_c<LEARN ? 1 : 4>(w, f->ustar);
+          move_c<LEARN ? 1 : 4>(w, f->D[i]);
           cont = true;
           break;
         }

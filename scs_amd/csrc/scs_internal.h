@@ -92,16 +92,17 @@ struct WaveHdr {
 
 // Search state of one vertex, packed so that the fields a rule or counter
 // update reads together come from one cache line.
-struct VState {
+struct __attribute__((aligned(16))) VState {
   u32 dR;     // |N(v) & R|
   u16 dC;     // |N(v) & C|
   u8 st;      // ST_R / ST_C / ST_X
   u8 queued;  // queued for removal in the current closure round
+  u64 acc;    // connection score being accumulated at the current node (0 between nodes)
 };
 
 // Offsets (bytes) of each per-wave array inside one wave's region.
 struct WaveLayout {
-  u64 vs, nmark, pos, rlist, acc, work, work2, front, aux, trail, sav, frames, clist, hist, hdr;
+  u64 vs, nmark, pos, rlist, work, work2, front, aux, trail, sav, frames, clist, hist, hdr;
   u64 stride;
 };
 
@@ -128,6 +129,8 @@ struct EngineGlobals {
   u64 dom_front;  // frontier vertices over all nodes that branched
   u64 dom_cand;   // of them, candidates whose adjacency was tested
   u64 dom_found;  // of them, dominated by ustar
+  u64 full_scans; // rule 1/2 closures that rescanned all of R
+  u64 full_rows;  // R vertices those rescans visited
   u64 tph[SCS_PHASES];  // wall-clock ticks (100 MHz) per search phase, all waves
 };
 
