@@ -108,6 +108,9 @@ struct Lds {
 
 // ---------------------------------------------------------------------------
 // Per-wave state view.
+// Candidates the MLP scores per node (learned order).
+#define LEARN_SHORTLIST 128u
+
 // Phases of the search whose wall-clock time is accounted separately.
 #define PH_REBUILD 0  // state rebuilt from a pooled task
 #define PH_RULES 1    // rule 3 and rule 1/2 closure, kl offers
@@ -810,20 +813,20 @@ __device__ __forceinline__ MlpFeat cand_features(const W& w, u32 x, u32 Cs, u32 
 
 // Highest MLP score over the frontier, ties to the lowest id (INV if no
 // finite score).
-__device__ u32 learned_ustar(W& w, u32 nf, u32 Cs, u32 kl) {
+__device__ u32 learned_ustar(W& w, const u32* cand, u32 nf, u32 Cs, u32 kl) {
   float bestf = -INFINITY;
   u32 bestx = INV;
 #pragma unroll 1
   for (u32 t0 = 0; t0 < nf; t0 += 16) {
     auto feat = [&](u32 row) {
       const u32 i = t0 + row;
-      if (i < nf) return cand_features(w, w.front[i], Cs, kl);
+      if (i < nf) return cand_features(w, cand[i], Cs, kl);
       return MlpFeat{0, 0, 0, 0, 0, 0.f};
     };
     const float sc = mlp_tile(w.model, feat, w.L->h1s);
     const u32 i = t0 + (LANE & 15u);
     if (LANE < 16 && i < nf) {
-      const u32 x = w.front[i];
+      const u32 x = cand[i];
       if (sc > bestf || (sc == bestf && x < bestx)) {
         bestf = sc;
         bestx = x;
@@ -1026,8 +1029,24 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   u32 us = wmin(best == gbest ? bid : INV);
   if (nf > 0 && w.trace && (nodeno % w.trace_stride) == 0) trace_node(w, nf, Cs, kl);
   if (LEARN && nf > 0 && us != INV && (w.flags & QF_LEARNED) && w.model) {
-    // learned order: any frontier vertex is a valid branching vertex
-    const u32 ul = learned_ustar(w, nf, Cs, kl);
+    // learned order: any frontier vertex is a valid branching vertex.  The
+    // MLP re-ranks a shortlist (plain score at least half the best, at most
+    // LEARN_SHORTLIST vertices) so that its cost does not grow with the
+    // frontier, which hubs make tens of thousands of vertices wide.
+    const u64 half = gbest >> 1;
+    if (LANE == 0) L->nk = 0;
+    wsync();
+    for (u32 i = LANE; i < nf; i += 64) {
+      const u32 x = w.front[i];
+      const VState v = w.vs[x];
+      if (v.acc + rterm(v.dR, w.dmax) >= half) {
+        const u32 p = atomicAdd(&L->nk, 1u);
+        if (p < LEARN_SHORTLIST) w.aux[p] = x;
+      }
+    }
+    wsync();
+    const u32 ns = min(bcast(L->nk), (u32)LEARN_SHORTLIST);
+    const u32 ul = learned_ustar(w, w.aux, ns, Cs, kl);
     if (ul != INV) us = ul;
   }
   if (nf == 0 || us == INV) {
