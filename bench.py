@@ -34,7 +34,8 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_EDGE = 4 + 7 + 6
 
 
-PHASES = ("rebuild", "rules", "bounds", "score", "doms", "move", "donate", "idle")
+PHASES = ("rebuild", "rules", "bounds", "score", "doms", "move", "donate", "idle", "closure_scan",
+          "closure_remove")
 
 
 def phase_shares(s0, s1):

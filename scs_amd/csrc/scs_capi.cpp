@@ -1037,7 +1037,7 @@ extern "C" int scs_engine_stats(const scs_engine* E, scs_stats* s) {
   s->donations = gl.donations;
   s->rebuilds = gl.rebuilds;
   s->edges = gl.edges;
-  for (int i = 0; i < 8; i++) s->phase_ticks[i] = gl.tph[i];
+  for (int i = 0; i < SCS_PHASES; i++) s->phase_ticks[i] = gl.tph[i];
   s->dom_front = gl.dom_front;
   s->dom_cand = gl.dom_cand;
   s->dom_found = gl.dom_found;

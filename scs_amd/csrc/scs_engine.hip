@@ -120,6 +120,8 @@ struct Lds {
 #define PH_MOVE 5     // applying children, backtracking (undo)
 #define PH_DONATE 6   // serialising a donated task
 #define PH_IDLE 7     // waiting for a task
+#define PH_CSCAN 8    // rule 1/2 closure: finding the vertices to remove
+#define PH_CREMOVE 9  // rule 1/2 closure: applying the removal batches
 
 __device__ __forceinline__ void ph_switch(Lds* L, u64& tp, u32& ph, u32 next) {
   const u64 t = wall_clock64();
@@ -414,18 +416,20 @@ struct Clo {
 };
 
 __device__ void dec_neighbour(W& w, u32 x, const Clo& c) {
-  u8 s = ST_R;
+  // live check first (an X vertex's counters are frozen and may be 0), then
+  // one 64-bit atomic on the record's first word {dR, dC, status, queued}:
+  // it decrements dR (>= 1: a live neighbour counts the removed R vertex)
+  // and returns dC and the status in the same round trip
   if (w.bm) {
     if (xbit(w, x)) return;
-    if (c.on) s = w.vs[x].st;
-  } else {
-    s = w.vs[x].st;
-    if (s == ST_X) return;
+  } else if (w.vs[x].st == ST_X) {
+    return;
   }
-  u32 old = atomicSub(&w.vs[x].dR, 1u);
+  const u64 old = atomicSub((unsigned long long*)&w.vs[x], 1ull);
+  const u32 s = (u32)(old >> 48) & 0xffu;
   if (c.on && s == ST_R) {
-    u32 dc = w.vs[x].dC;
-    if (dc <= c.kl && old + dc == c.kl + 1) {
+    const u32 dc = (u32)(old >> 32) & 0xffffu;
+    if (dc <= c.kl && (u32)old + dc == c.kl + 1) {
       u32* wp = (u32*)((uintptr_t)(&w.vs[x].queued) & ~(uintptr_t)3);
       u32 sh = ((uintptr_t)(&w.vs[x].queued) & 3) * 8;
       u32 o2 = atomicOr(wp, 1u << sh);
@@ -514,9 +518,15 @@ __device__ __forceinline__ void remove_batch(W& w, const u32* list, u32 m, const
     // phase 4: save and recount dR of the live vertices (R list, then C)
     u32* S = w.sav + w.nsav;
     const u32 nR = w.nR;
+    const bool bm = w.bm;
+    if (bm) {
+      // tb = C: "x is R" is two LDS bit tests
+      for (u32 i = LANE; i < w.nC; i += 64) tset(w, w.clist[i]);
+      wsync();
+    }
     scan_adj<u32>(
         w, live_after, [&](u32 i) { return i < nR ? w.rlist[i] : w.clist[i - nR]; },
-        [&](u32 x) { return (u32)(w.vs[x].st == ST_R); },
+        [&](u32 x) { return (u32)(bm ? !(xbit(w, x) || tbit(w, x)) : w.vs[x].st == ST_R); },
         [&](u32 i, u32 v, u32 r) {
           S[2 * i] = v;
           S[2 * i + 1] = w.vs[v].dR;
@@ -528,6 +538,10 @@ __device__ __forceinline__ void remove_batch(W& w, const u32* list, u32 m, const
             if (!((o2 >> sh) & 0xffu)) c.next[atomicAdd(c.cnt, 1u)] = v;
           }
         });
+    if (bm) {
+      wsync();
+      for (u32 i = LANE; i < w.nC; i += 64) tclr(w, w.clist[i]);
+    }
     if (LANE == 0) S[2 * live_after] = live_after;
     w.nsav += 2 * live_after + 1;
     wsync();
@@ -670,10 +684,12 @@ __device__ void try_push(W& w, u32 x, u32 Cs, u32 ubD, u32 kl, u32* list,
 // (|C| grew), or everything when kl changed.  Each round removes all
 // queued vertices as one batch and queues the neighbours it pushes below
 // the rule-1 threshold.  Returns true if anything was removed.
-__device__ __forceinline__ bool r12(W& w, u32 from, bool full, u32 kl) {
+__device__ __forceinline__ bool r12(W& w, u32 from, bool full, u32 kl, u64& tp, u32& ph) {
   Lds* L = w.L;
   const u32 Cs = w.nC;
   const u32 ubD = ubd_node(kl, w.h);
+  const u32 ph0 = ph;
+  ph_switch(L, tp, ph, PH_CSCAN);
   if (LANE == 0) L->cnt = 0;
   wsync();
   if (!full && (w.flags & QF_RED1)) {
@@ -718,6 +734,7 @@ __device__ __forceinline__ bool r12(W& w, u32 from, bool full, u32 kl) {
   wsync();
   u32 cnt = bcast(L->cnt);
   const bool removed = cnt > 0;
+  ph_switch(L, tp, ph, PH_CREMOVE);
   u32* cur = w.work;
   u32* nxt = w.work2;
   while (cnt) {
@@ -734,6 +751,7 @@ __device__ __forceinline__ bool r12(W& w, u32 from, bool full, u32 kl) {
     cur = nxt;
     nxt = t;
   }
+  ph_switch(L, tp, ph, ph0);
   return removed;
 }
 
@@ -931,7 +949,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
       }
       continue;
     }
-    const bool removed = r12(w, from, full, kl);
+    const bool removed = r12(w, from, full, kl, tp, ph);
     from = w.ntrail;
     full = false;
     if (removed && (w.flags & QF_RED3) && tight_exists(w, kl)) continue;

@@ -14,7 +14,7 @@ typedef int64_t i64;
 #define SCS_WAVE 64
 #define SCS_FRAME_DOMS SCS_MAX_DOMS
 #define SCS_HCAP (SCS_MAX_H + 2)
-#define SCS_PHASES 8
+#define SCS_PHASES 10
 
 // vertex status inside a search state
 #define ST_R 0
@@ -99,6 +99,12 @@ struct __attribute__((aligned(16))) VState {
   u8 queued;  // queued for removal in the current closure round
   u64 acc;    // connection score being accumulated at the current node (0 between nodes)
 };
+// The engine updates {dR, dC, st, queued} as one little-endian 64-bit word
+// (dR in bits 0-31, dC in 32-47, st in 48-55, queued in 56-63).
+static_assert(sizeof(VState) == 16, "VState is 16 bytes");
+static_assert(__builtin_offsetof(VState, dR) == 0 && __builtin_offsetof(VState, dC) == 4 &&
+                  __builtin_offsetof(VState, st) == 6 && __builtin_offsetof(VState, queued) == 7,
+              "VState word layout");
 
 // Offsets (bytes) of each per-wave array inside one wave's region.
 struct WaveLayout {
