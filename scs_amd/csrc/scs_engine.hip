@@ -415,17 +415,11 @@ struct Clo {
   u32* cnt;  // LDS counter of next
 };
 
-__device__ void dec_neighbour(W& w, u32 x, const Clo& c) {
-  // live check first (an X vertex's counters are frozen and may be 0), then
-  // one 64-bit atomic on the record's first word {dR, dC, status, queued}:
-  // it decrements dR (>= 1: a live neighbour counts the removed R vertex)
-  // and returns dC and the status in the same round trip
-  if (w.bm) {
-    if (xbit(w, x)) return;
-  } else if (w.vs[x].st == ST_X) {
-    return;
-  }
-  const u64 old = atomicSub((unsigned long long*)&w.vs[x], 1ull);
+__device__ __forceinline__ bool dec_live(const W& w, u32 x) {
+  return w.bm ? !xbit(w, x) : w.vs[x].st != ST_X;
+}
+// old: the record's first word before the decrement of a live neighbour x
+__device__ __forceinline__ void dec_check(W& w, u32 x, u64 old, const Clo& c) {
   const u32 s = (u32)(old >> 48) & 0xffu;
   if (c.on && s == ST_R) {
     const u32 dc = (u32)(old >> 32) & 0xffffu;
@@ -436,6 +430,61 @@ __device__ void dec_neighbour(W& w, u32 x, const Clo& c) {
       if (!((o2 >> sh) & 0xffu)) c.next[atomicAdd(c.cnt, 1u)] = x;
     }
   }
+}
+// Live check first (an X vertex's counters are frozen and may be 0), then
+// one 64-bit atomic on the record's first word {dR, dC, status, queued}: it
+// decrements dR (>= 1: a live neighbour counts the removed R vertex) and
+// returns dC and the status in the same round trip.
+__device__ __forceinline__ void dec_neighbour(W& w, u32 x, const Clo& c) {
+  if (!dec_live(w, x)) return;
+  const u64 old = atomicSub((unsigned long long*)&w.vs[x], 1ull);
+  dec_check(w, x, old, c);
+}
+
+// Decrements dR of every live neighbour of list[0..m) (load-balanced like
+// scan_adj); on the per-lane path four decrements are in flight before
+// their returned words are checked.
+__device__ __forceinline__ void dec_neighbours(W& w, const u32* list, u32 m, const Clo& c) {
+  u32 ed = 0;
+  for (u32 c0 = 0; c0 < m; c0 += 64) {
+    const u32 i = c0 + LANE;
+    u32 v = 0, b = 0, e = 0;
+    if (i < m) {
+      v = list[i];
+      b = w.off[v];
+      e = w.off[v + 1];
+    }
+    const bool big = i < m && (e - b) > SEQ_DEG;
+    if (i < m && !big) {
+      u32 j = b;
+      for (; j + 4 <= e; j += 4) {
+        u32 x[4];
+        bool lv[4];
+        u64 od[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) x[k] = w.adj[j + k];
+#pragma unroll
+        for (int k = 0; k < 4; k++) lv[k] = dec_live(w, x[k]);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          od[k] = lv[k] ? atomicSub((unsigned long long*)&w.vs[x[k]], 1ull) : 0ull;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (lv[k]) dec_check(w, x[k], od[k], c);
+      }
+      for (; j < e; j++) dec_neighbour(w, w.adj[j], c);
+      ed += e - b;
+    }
+    u64 bm = __ballot(big);
+    while (bm) {
+      const int src = __ffsll((unsigned long long)bm) - 1;
+      bm &= bm - 1;
+      const u32 bb = __shfl(b, src), be = __shfl(e, src);
+      for (u32 j = bb + LANE; j < be; j += 64) dec_neighbour(w, w.adj[j], c);
+      if (LANE == 0) ed += be - bb;
+    }
+  }
+  w.edges += wsum(ed);
 }
 
 // Removes the m vertices of list (all currently R) as one trail batch.
@@ -455,13 +504,7 @@ __device__ __forceinline__ void remove_batch(W& w, const u32* list, u32 m, const
   // phase 1 (exact mode): decrement the live neighbours (batch included)
   if (!recount) {
     if (m >= 8) {
-      scan_adj<u32>(
-          w, m, [&](u32 i) { return list[i]; },
-          [&](u32 x) {
-            dec_neighbour(w, x, c);
-            return 0u;
-          },
-          [&](u32, u32, u32) {});
+      dec_neighbours(w, list, m, c);
     } else {
       for (u32 i = 0; i < m; i++) {
         u32 v = list[i];
