@@ -164,9 +164,10 @@ typedef struct scs_stats {
   uint64_t edges;          /* adjacency entries scanned by the search */
   double core_us;          /* device time of the core decomposition */
   uint64_t prep_batches;   /* batched preprocessing passes (up to 64 queries each) */
-  uint64_t phase_ticks[10]; /* search wall-clock ticks (100 MHz) summed over waves:
-                              rebuild, rules (other), bounds, score, doms, move,
-                              donate, idle, closure scan, closure removals */
+  uint64_t phase_ticks[12]; /* search wall-clock ticks (100 MHz) summed over waves:
+                              rebuild, rules (other), bounds, score (scatter and
+                              shortlist), doms, move, donate, idle, closure scan,
+                              closure removals, score argmax, score clearing */
   uint64_t dom_front;      /* frontier vertices summed over branching nodes */
   uint64_t dom_cand;       /* frontier vertices tested for domination */
   uint64_t dom_found;      /* tested vertices found dominated by ustar */

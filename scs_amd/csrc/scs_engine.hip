@@ -122,6 +122,8 @@ struct Lds {
 #define PH_IDLE 7     // waiting for a task
 #define PH_CSCAN 8    // rule 1/2 closure: finding the vertices to remove
 #define PH_CREMOVE 9  // rule 1/2 closure: applying the removal batches
+#define PH_ARGMAX 10  // argmax of the connection score over the frontier
+#define PH_CLEAR 11   // clearing the frontier's scores
 
 __device__ __forceinline__ void ph_switch(Lds* L, u64& tp, u32& ph, u32 next) {
   const u64 t = wall_clock64();
@@ -1025,33 +1027,63 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     for (u32 i = LANE; i < Cs; i += 64) tset(w, w.clist[i]);
     wsync();
   }
-  for (u32 i = 0; i < Cs; i++) {
-    u32 u = w.clist[i];
-    u64 r = recip32(w.vs[u].dC);
-    const u32 b = w.off[u], e = w.off[u + 1];
-    // four 64-entry chunks per step: loads, tests and atomics of the four
-    // are issued before any returned value is used (the learned variant
-    // keeps one chunk per step: it has no registers to spare)
-    constexpr int U = LEARN ? 1 : 4;
-    for (u32 j = b + LANE; j < e; j += 64 * U) {
+  // The adjacency lists of C, concatenated: 64 C vertices at a time, each
+  // lane loads one vertex's setup (score term, list bounds) and the wave
+  // walks the concatenated lists 64 entries per chunk; the owner of an
+  // entry is found by binary search over the lanes' exclusive degree sums.
+  // U chunks per step keep their loads, tests and atomics in flight
+  // together (the learned variant has no registers to spare for U > 1).
+  constexpr int U = LEARN ? 1 : 4;
+  for (u32 c0 = 0; c0 < Cs; c0 += 64) {
+    const u32 ci = c0 + LANE;
+    u32 ub = 0, ud = 0;
+    u64 ur = 0;
+    if (ci < Cs) {
+      const u32 u = w.clist[ci];
+      ur = recip32(w.vs[u].dC);
+      ub = w.off[u];
+      ud = w.off[u + 1] - ub;
+    }
+    u32 inc = ud;
+    for (int o = 1; o < 64; o <<= 1) {
+      const u32 t = __shfl_up(inc, o);
+      if (LANE >= (u32)o) inc += t;
+    }
+    const u32 tot = __shfl(inc, 63);
+    const u32 exc = inc - ud;
+    for (u32 k0 = 0; k0 < tot; k0 += 64 * U) {
       u32 x[U];
+      u64 rr[U];
       bool ok[U];
 #pragma unroll
-      for (int k = 0; k < U; k++) x[k] = j + 64 * k < e ? w.adj[j + 64 * k] : 0u;
+      for (int k = 0; k < U; k++) {
+        const u32 kk = k0 + 64 * k + LANE;
+        const u32 q = kk < tot ? kk : tot - 1;
+        u32 lo = 0;
+        for (u32 step = 32; step > 0; step >>= 1) {
+          const u32 probe = lo + step;
+          const u32 pe = __shfl(exc, (int)(probe < 64 ? probe : 63));
+          if (probe < 64 && pe <= q) lo = probe;
+        }
+        const u32 ob = __shfl(ub, (int)lo), oe = __shfl(exc, (int)lo);
+        const u32 rlo = __shfl((u32)ur, (int)lo), rhi = __shfl((u32)(ur >> 32), (int)lo);
+        rr[k] = ((u64)rhi << 32) | rlo;
+        x[k] = kk < tot ? w.adj[ob + (q - oe)] : 0u;
+      }
 #pragma unroll
       for (int k = 0; k < U; k++)
-        ok[k] = j + 64 * k < e &&
+        ok[k] = k0 + 64 * k + LANE < tot &&
                 (w.bm ? !(xbit(w, x[k]) || tbit(w, x[k])) : w.vs[x[k]].st == ST_R);
       u64 old[U];
 #pragma unroll
       for (int k = 0; k < U; k++)
-        old[k] = ok[k] ? atomicAdd((unsigned long long*)&w.vs[x[k]].acc, (unsigned long long)r)
+        old[k] = ok[k] ? atomicAdd((unsigned long long*)&w.vs[x[k]].acc, (unsigned long long)rr[k])
                        : 1ull;
 #pragma unroll
       for (int k = 0; k < U; k++)
         if (old[k] == 0) w.front[atomicAdd(&L->nfront, 1u)] = x[k];
     }
-    w.edges += e - b;
+    w.edges += tot;
   }
   wsync();
   if (w.bm) {
@@ -1060,6 +1092,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   }
   const u32 nf = bcast(L->nfront);
   // argmax score, ties to the lowest id
+  ph_switch(L, tp, ph, PH_ARGMAX);
   u64 best = 0;
   u32 bid = INV;
   auto consider = [&](u32 x, u64 s) {
@@ -1088,6 +1121,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   }
   const u64 gbest = wmax64(best);
   u32 us = wmin(best == gbest ? bid : INV);
+  ph_switch(L, tp, ph, PH_SCORE);
   if (nf > 0 && w.trace && (nodeno % w.trace_stride) == 0) trace_node(w, nf, Cs, kl);
   if (LEARN && nf > 0 && us != INV && (w.flags & QF_LEARNED) && w.model) {
     // learned order: any frontier vertex is a valid branching vertex.  The
@@ -1111,6 +1145,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     if (ul != INV) us = ul;
   }
   if (nf == 0 || us == INV) {
+    ph_switch(L, tp, ph, PH_CLEAR);
     clear_acc(w, nf);
     return o;
   }
@@ -1208,6 +1243,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     }
     nD = want;
   }
+  ph_switch(L, tp, ph, PH_CLEAR);
   clear_acc(w, nf);
   o.branch = 1;
   o.ustar = us;

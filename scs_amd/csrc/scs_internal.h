@@ -14,7 +14,7 @@ typedef int64_t i64;
 #define SCS_WAVE 64
 #define SCS_FRAME_DOMS SCS_MAX_DOMS
 #define SCS_HCAP (SCS_MAX_H + 2)
-#define SCS_PHASES 10
+#define SCS_PHASES 12
 
 // vertex status inside a search state
 #define ST_R 0

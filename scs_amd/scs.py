@@ -61,7 +61,7 @@ class Stats(C.Structure):
                 ("engine_us", C.c_double), ("prep_us", C.c_double), ("heu_us", C.c_double),
                 ("prep_bytes", C.c_uint64), ("epochs", C.c_uint64), ("waves", C.c_uint32),
                 ("kcap", C.c_uint32), ("edges", C.c_uint64), ("core_us", C.c_double),
-                ("prep_batches", C.c_uint64), ("phase_ticks", C.c_uint64 * 10),
+                ("prep_batches", C.c_uint64), ("phase_ticks", C.c_uint64 * 12),
                 ("dom_front", C.c_uint64), ("dom_cand", C.c_uint64), ("dom_found", C.c_uint64),
                 ("full_scans", C.c_uint64), ("full_rows", C.c_uint64)]
 
