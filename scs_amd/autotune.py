@@ -5,6 +5,7 @@ workload's query stream and keeps the configuration with the highest
 queries/s:
   waves_per_cu   resident search waves per CU (the reference's grid shape,
                  main.cu:543-559, chosen here by measurement)
+  partition_size per-wave kept-set capacity (partitionSize, main.cu:481)
   limit_queries  resident query slots (the reference's limitQueries)
   copy_limit     donation threshold: busy waves hand over work while fewer
                  than copy_limit x waves tasks are pooled (copyLimit)
@@ -31,10 +32,12 @@ import numpy as np
 from . import configs, scs
 
 GRID = {
-    "waves_per_cu": [4, 8, 16],
+    "waves_per_cu": [2, 4, 8],
     "limit_queries": [512, 2048],
-    "copy_limit": [0.02, 0.5],
-    "time_slice_us": [5000],
+    "copy_limit": [0.1, 0.5],
+    "time_slice_us": [60000, 200000],
+    "limit_tasks": [0],       # task pool slots (the reference's limitTasks); 0 = 2 x waves
+    "partition_size": [0],    # per-wave kept-set capacity (partitionSize); 0 = fitted to memory
 }
 
 
