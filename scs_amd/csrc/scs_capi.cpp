@@ -44,14 +44,15 @@ hipError_t scs_launch_ms_bscan(u32 nb, u32 G, u32* gcnt, u32* tot, unsigned long
 hipError_t scs_launch_ms_scatter(u32 n, u32 G, const u64* vis, u64 valid, const u32* gcnt,
                                  const u32* core, const u16* dist16, const MsBatch* B, u32* locid,
                                  u32* qloc, unsigned long long* bytes, hipStream_t s);
-hipError_t scs_launch_ms_degree(const i64* off, const int32_t* adj, u32 n, const u64* vis,
-                                u64 valid, const u32* locid, const MsBatch* B,
+hipError_t scs_launch_ms_degree(const i64* off, const int32_t* adj, const MsUnit* units,
+                                u32 nunits, const MsHub* hubs, u32 nhubs, const u64* vis,
+                                u64 valid, const u32* locid, const MsBatch* B, u32* hubcnt,
                                 unsigned long long* bytes, hipStream_t s);
-hipError_t scs_launch_ms_scan_loff(u32 nb, const MsBatch* B, u32* tot, u64 valid,
-                                   unsigned long long* bytes, hipStream_t s);
-hipError_t scs_launch_ms_fill(const i64* off, const int32_t* adj, u32 n, const u64* vis, u64 valid,
-                              const u32* locid, const MsBatch* B, unsigned long long* bytes,
-                              hipStream_t s);
+hipError_t scs_launch_ms_scan_loff(u32 nb, const MsBatch* B, u32* tot, u64 valid, u32 cmax,
+                                   u32* csum, unsigned long long* bytes, hipStream_t s);
+hipError_t scs_launch_ms_fill(const i64* off, const int32_t* adj, const MsUnit* units, u32 nunits,
+                              const u64* vis, u64 valid, const u32* locid, const MsBatch* B,
+                              const u32* hubcnt, unsigned long long* bytes, hipStream_t s);
 hipError_t scs_launch_epoch(const EngineArgs* A, hipStream_t s);
 hipError_t scs_launch_admit(const EngineArgs* A, const u32* slots, u32 n, u32* failed,
                             hipStream_t s);
@@ -302,6 +303,11 @@ struct scs_engine {
   u16* d_dist16 = nullptr;   // [n][64] BFS distance per query
   u32* d_locid64 = nullptr;  // [n][64] local id per query
   u32* d_gcnt = nullptr;     // [ceil(n/64)][64] group counts, then bases
+  MsUnit* d_units = nullptr;  // adjacency units of every vertex
+  MsHub* d_hubs = nullptr;    // vertices with more than one unit
+  u32 nunits = 0, nhubs = 0;
+  u32* d_hubcnt = nullptr;   // [hub units][64] partial counts, then unit offsets
+  u32* d_csum = nullptr;     // [64][cmax] loff chunk sums, then chunk bases
   u32* d_flags = nullptr;    // one per BFS level: something was found
   u32* d_tot = nullptr;      // K[64] | nnz[64] | qloc[64]
   MsBatch* d_batch = nullptr;
@@ -446,12 +452,44 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   if (hipMalloc(&E->d_dist16, sizeof(u16) * 64 * (u64)n1) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_locid64, sizeof(u32) * 64 * (u64)n1) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_gcnt, sizeof(u32) * 64 * G) != hipSuccess) return fail(SCS_ENOMEM);
+  {
+    // adjacency units: one per vertex, or ceil(deg / MS_UNIT) for a hub
+    std::vector<MsUnit> units;
+    std::vector<MsHub> hubs;
+    units.reserve(n1);
+    u32 hub_units = 0;
+    for (u32 v = 0; v < n; v++) {
+      const u64 deg = (u64)(E->g->hg.off[v + 1] - E->g->hg.off[v]);
+      const u64 nu = (deg + MS_UNIT - 1) / MS_UNIT;
+      if (nu <= 1) {
+        units.push_back(MsUnit{v, 0, MS_NOHUB, 0});
+        continue;
+      }
+      hubs.push_back(MsHub{v, hub_units, (u32)nu, 0});
+      for (u32 si = 0; si < nu; si++) units.push_back(MsUnit{v, si, hub_units + si, 0});
+      hub_units += (u32)nu;
+    }
+    E->nunits = (u32)units.size();
+    E->nhubs = (u32)hubs.size();
+    if (hipMalloc(&E->d_units, sizeof(MsUnit) * std::max<size_t>(units.size(), 1)) != hipSuccess ||
+        hipMalloc(&E->d_hubs, sizeof(MsHub) * std::max<size_t>(hubs.size(), 1)) != hipSuccess ||
+        hipMalloc(&E->d_hubcnt, sizeof(u32) * 64 * std::max<u64>(hub_units, 1)) != hipSuccess ||
+        hipMalloc(&E->d_csum, sizeof(u32) * 64 * ((u64)E->kcap / 4096 + 2)) != hipSuccess)
+      return fail(SCS_ENOMEM);
+    if (!units.empty() && hipMemcpy(E->d_units, units.data(), sizeof(MsUnit) * units.size(),
+                                    hipMemcpyHostToDevice) != hipSuccess)
+      return fail(SCS_EHIP);
+    if (!hubs.empty() && hipMemcpy(E->d_hubs, hubs.data(), sizeof(MsHub) * hubs.size(),
+                                   hipMemcpyHostToDevice) != hipSuccess)
+      return fail(SCS_EHIP);
+  }
   if (hipMalloc(&E->d_flags, sizeof(u32) * SCS_MS_LEVELS) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_tot, sizeof(u32) * 192) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_batch, sizeof(MsBatch)) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipHostMalloc(&E->h_batch, sizeof(MsBatch)) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_admit, sizeof(u32) * 2 * nq) != hipSuccess) return fail(SCS_ENOMEM);
-  if (hipMalloc(&E->d_prepbytes, sizeof(unsigned long long)) != hipSuccess) return fail(SCS_ENOMEM);
+  const size_t prepbytes_len = sizeof(unsigned long long) * MS_BYTE_SLOTS * MS_BYTE_STRIDE;
+  if (hipMalloc(&E->d_prepbytes, prepbytes_len) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_arena, p.mem_budget) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipHostMalloc(&E->h_tot, sizeof(u32) * 192) != hipSuccess) return fail(SCS_ENOMEM);
   E->arena.init(E->d_arena, p.mem_budget);
@@ -459,7 +497,7 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   if (hipMemset(E->d_wave, 0, lay.stride * E->nwaves) != hipSuccess) return fail(SCS_EHIP);
   if (hipMemset(E->d_pool_pos, 0, sizeof(u64) * 2) != hipSuccess) return fail(SCS_EHIP);
   if (hipMemset(E->d_glob, 0, sizeof(EngineGlobals)) != hipSuccess) return fail(SCS_EHIP);
-  if (hipMemset(E->d_prepbytes, 0, sizeof(unsigned long long)) != hipSuccess) return fail(SCS_EHIP);
+  if (hipMemset(E->d_prepbytes, 0, prepbytes_len) != hipSuccess) return fail(SCS_EHIP);
   if (scs_launch_pool_init(E->d_pool_seq, nslots, E->s) != hipSuccess) return fail(SCS_EHIP);
   if (hipStreamSynchronize(E->s) != hipSuccess) return fail(SCS_EHIP);
   for (hipEvent_t* ev : {&E->ev0, &E->ev1, &E->pe0, &E->pe1})
@@ -554,7 +592,8 @@ extern "C" void scs_engine_destroy(scs_engine* E) {
   if (E->ps) (void)hipStreamSynchronize(E->ps);
   void* ptrs[] = {E->d_slots, E->d_witness, E->d_wave, E->d_pool_seq, E->d_pool_data,
                   E->d_pool_pos, E->d_glob, E->d_elig, E->d_vis, E->d_fr, E->d_nf, E->d_dist16,
-                  E->d_locid64, E->d_gcnt, E->d_flags, E->d_tot, E->d_batch, E->d_admit,
+                  E->d_locid64, E->d_gcnt, E->d_units, E->d_hubs, E->d_hubcnt, E->d_csum,
+                  E->d_flags, E->d_tot, E->d_batch, E->d_admit,
                   E->d_prepbytes, E->d_arena, E->d_model, E->d_trace, E->d_trace_cnt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -713,6 +752,7 @@ static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32*
   HIPCHK(hipMemcpyAsync(E->h_tot, E->d_tot, sizeof(u32) * 64, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   u64 valid = 0;
+  u32 maxK = 0;
   for (u32 b = 0; b < cnt; b++) {
     const u32 i = ord[b];
     HostSlot& hs = E->slots[sis[i]];
@@ -738,14 +778,17 @@ static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32*
     B.P[b].lcore = A + 2ull * K + 1;
     B.P[b].lorig = A + 3ull * K + 1;
     valid |= 1ull << b;
+    maxK = std::max(maxK, K);
   }
   if (valid) {
     HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
     HIPCHK(scs_launch_ms_scatter(n, G, E->d_vis, valid, E->d_gcnt, E->g->d_core, E->d_dist16,
                                  E->d_batch, E->d_locid64, E->d_tot + 128, E->d_prepbytes, s));
-    HIPCHK(scs_launch_ms_degree(off, adj, n, E->d_vis, valid, E->d_locid64, E->d_batch,
-                                E->d_prepbytes, s));
-    HIPCHK(scs_launch_ms_scan_loff(cnt, E->d_batch, E->d_tot, valid, E->d_prepbytes, s));
+    HIPCHK(scs_launch_ms_degree(off, adj, E->d_units, E->nunits, E->d_hubs, E->nhubs, E->d_vis,
+                                valid, E->d_locid64, E->d_batch, E->d_hubcnt, E->d_prepbytes, s));
+    HIPCHK(scs_launch_ms_scan_loff(cnt, E->d_batch, E->d_tot, valid,
+                                   std::max<u32>(1, (maxK + 4095) / 4096), E->d_csum,
+                                   E->d_prepbytes, s));
     HIPCHK(hipMemcpyAsync(E->h_tot + 64, E->d_tot + 64, sizeof(u32) * 128, hipMemcpyDeviceToHost,
                           s));
     HIPCHK(hipStreamSynchronize(s));
@@ -768,8 +811,8 @@ static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32*
     }
     if (valid) {
       HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
-      HIPCHK(scs_launch_ms_fill(off, adj, n, E->d_vis, valid, E->d_locid64, E->d_batch,
-                                E->d_prepbytes, s));
+      HIPCHK(scs_launch_ms_fill(off, adj, E->d_units, E->nunits, E->d_vis, valid, E->d_locid64,
+                                E->d_batch, E->d_hubcnt, E->d_prepbytes, s));
     }
   }
   HIPCHK(hipEventRecord(E->pe1, s));
@@ -1044,9 +1087,11 @@ extern "C" int scs_engine_stats(const scs_engine* E, scs_stats* s) {
   s->full_scans = gl.full_scans;
   s->full_rows = gl.full_rows;
   s->core_us = E->g->core_us;
-  unsigned long long pb = 0;
-  HIPCHK(hipMemcpy(&pb, E->d_prepbytes, sizeof(pb), hipMemcpyDeviceToHost));
-  s->prep_bytes = pb;
+  std::vector<unsigned long long> pb((size_t)MS_BYTE_SLOTS * MS_BYTE_STRIDE);
+  HIPCHK(hipMemcpy(pb.data(), E->d_prepbytes, sizeof(unsigned long long) * pb.size(),
+                   hipMemcpyDeviceToHost));
+  s->prep_bytes = 0;
+  for (u32 i = 0; i < MS_BYTE_SLOTS; i++) s->prep_bytes += pb[(size_t)i * MS_BYTE_STRIDE];
   return SCS_OK;
 }
 

@@ -160,6 +160,31 @@ struct MsBatch {
   MsQuery P[64];
 };
 
+// The adjacency of every vertex cut into units of at most MS_UNIT entries,
+// once per graph: the local degree and adjacency passes take one unit per
+// wave, so a hub's list is spread over many waves.  A vertex with more than
+// one unit (a hub) numbers its units h0 .. h0+nu-1 for the per-unit partial
+// counts; a single-unit vertex has h == MS_NOHUB.
+#define MS_UNIT 256
+#define MS_NOHUB 0xFFFFFFFFu
+struct MsUnit {
+  u32 u;  // vertex
+  u32 s;  // unit index within the vertex: entries off[u] + s*MS_UNIT ..
+  u32 h;  // hub unit number, or MS_NOHUB
+  u32 pad;
+};
+struct MsHub {
+  u32 u;
+  u32 h0;  // first hub unit number
+  u32 nu;  // units
+  u32 pad;
+};
+
+// Algorithmic-byte counter of the preprocessing kernels: MS_BYTE_SLOTS
+// u64 counters MS_BYTE_STRIDE words apart, summed by the host.
+#define MS_BYTE_SLOTS 64
+#define MS_BYTE_STRIDE 16
+
 struct EngineArgs {
   const MlpDev* model;  // branching MLP (queries with QF_LEARNED)
   u32* trace;           // search-trace rows (8 words each), or null

@@ -55,13 +55,53 @@ __device__ __forceinline__ u64 ms_shfl64(u64 x, int src) {
   const u32 lo = __shfl((u32)x, src), hi = __shfl((u32)(x >> 32), src);
   return ((u64)hi << 32) | lo;
 }
+// One add per wave into one of MS_BYTE_SLOTS counters, each on its own
+// 128-byte line: adds to a single address serialise at the L2 (about 12 ns
+// each), which had made the count, not the work, bound the wide launches.
 __device__ __forceinline__ void ms_count_bytes(unsigned long long* ctr, u64 b) {
   b = ms_sum(b);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(ctr, (unsigned long long)b);
+  const u32 slot = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) & (MS_BYTE_SLOTS - 1);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(ctr + slot * MS_BYTE_STRIDE, (unsigned long long)b);
 }
 __device__ __forceinline__ u64 ms_below() {
   const u32 lane = threadIdx.x & 63;
   return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+__device__ __forceinline__ u64 ms_shfl_xor64(u64 x, int m) {
+  const u32 lo = __shfl_xor((u32)x, m), hi = __shfl_xor((u32)(x >> 32), m);
+  return ((u64)hi << 32) | lo;
+}
+// 64 x 64 bit transpose across the wave: on entry lane j holds row j, on
+// exit lane b holds column b (bit j of lane b = bit b of row j).  Six
+// butterfly stages; at stage j the pair (lane, lane ^ j) swaps the j x j
+// blocks off the diagonal.
+__device__ __forceinline__ u64 ms_transpose(u64 a) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 M[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                    0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const int j = 32 >> s;
+    const u64 p = ms_shfl_xor64(a, j);
+    const u64 up = ((p >> j) ^ a) & M[s];           // lane has bit j set
+    const u64 dn = (((a >> j) ^ p) & M[s]) << j;    // lane has bit j clear
+    a ^= (lane & (u32)j) ? up : dn;
+  }
+  return a;
+}
+// Inclusive sum over the wave's lanes.
+__device__ __forceinline__ u32 ms_wave_incl(u32 x) {
+  const u32 lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const u32 y = __shfl_up(x, o);
+    if (lane >= (u32)o) x += y;
+  }
+  return x;
+}
+// The wave number as a wave-uniform value.
+__device__ __forceinline__ u32 ms_wave_id() {
+  return __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
 }
 
 // elig/vis/fr for every vertex; level flags cleared.
@@ -147,19 +187,30 @@ __global__ void __launch_bounds__(256) ms_level(const i64* __restrict__ off,
     }
     const bool big = want && (e0 - b0) > MS_HUB;
     bool active = want && !big;
+    // 64 entries per segment step: each lane issues its four adjacency
+    // loads, then its four frontier gathers, so a step costs two memory
+    // round trips
     u64 acc = 0;
     i64 j = b0 + sl;
     while (__ballot(active)) {
       u64 x = 0;
-      if (active && j < e0) {
-        x = fr[(u32)adj[j]] & want;
-        nbytes += 12;
+      if (active) {
+        u32 w[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) w[t] = j + 16 * t < e0 ? (u32)adj[j + 16 * t] : 0u;
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          if (j + 16 * t < e0) {
+            x |= fr[w[t]];
+            nbytes += 12;
+          }
+        x &= want;
       }
       acc |= ms_seg_or(x);
-      j += 16;
+      j += 64;
       if (active && (acc == want || j - (i64)sl >= e0)) active = false;
     }
-    // hubs: the whole wave, 64 entries per step
+    // hubs: the whole wave, 256 entries per step
     u64 bm = __ballot(big && sl == 0);
     while (bm) {
       const int src = __ffsll((unsigned long long)bm) - 1;
@@ -167,13 +218,18 @@ __global__ void __launch_bounds__(256) ms_level(const i64* __restrict__ off,
       const u64 bwant = ms_shfl64(want, src);
       const i64 bb = (i64)ms_shfl64((u64)b0, src), be = (i64)ms_shfl64((u64)e0, src);
       u64 bacc = 0;
-      for (i64 jj = bb; jj < be; jj += 64) {
+      for (i64 jj = bb + lane; jj - (i64)lane < be; jj += 256) {
+        u32 w[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) w[t] = jj + 64 * t < be ? (u32)adj[jj + 64 * t] : 0u;
         u64 x = 0;
-        if (jj + lane < be) {
-          x = fr[(u32)adj[jj + lane]] & bwant;
-          nbytes += 12;
-        }
-        bacc |= ms_or(x);
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          if (jj + 64 * t < be) {
+            x |= fr[w[t]];
+            nbytes += 12;
+          }
+        bacc |= ms_or(x & bwant);
         if (bacc == bwant) break;
       }
       // hand the result to the hub's segment
@@ -198,7 +254,11 @@ __global__ void __launch_bounds__(256) ms_level(const i64* __restrict__ off,
       }
     }
   }
-  if (__ballot(found) && lane == 0) atomicOr(flag_cur, 1u);
+  // set the level flag once: later waves read it from the L2 and skip the
+  // atomic, so the finders do not queue on one address
+  if (__ballot(found) && lane == 0 &&
+      __hip_atomic_load(flag_cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    atomicOr(flag_cur, 1u);
   ms_count_bytes(bytes, nbytes);
 }
 
@@ -208,62 +268,39 @@ __global__ void __launch_bounds__(256) ms_count(u32 n, u32 G, const u64* __restr
                                                 u64 valid, u32* __restrict__ gcnt,
                                                 unsigned long long* bytes) {
   const u32 lane = threadIdx.x & 63;
-  const u32 wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const u32 wid = ms_wave_id();
   const u32 nw = (gridDim.x * blockDim.x) >> 6;
   u64 nbytes = 0;
   for (u32 g = wid; g < G; g += nw) {
     const u64 v = (u64)g * 64 + lane;
     const u64 m = v < n ? vis[v] & valid : 0;
     if (v < n) nbytes += 8;
-    u64 any = ms_or(m);
-    u32 mine = 0;
-    while (any) {
-      const int b = __ffsll((unsigned long long)any) - 1;
-      any &= any - 1;
-      const u64 bal = __ballot((m >> b) & 1ull);
-      if ((int)lane == b) mine = (u32)__popcll(bal);
-    }
-    gcnt[(u64)g * 64 + lane] = mine;
+    // lane b: the group's vertices kept by query b
+    gcnt[(u64)g * 64 + lane] = (u32)__popcll(ms_transpose(m));
     nbytes += 4;
   }
   ms_count_bytes(bytes, nbytes);
 }
 
-// Exclusive block-wide scan of one value per thread (256 threads).
-__device__ __forceinline__ u32 ms_block_excl(u32 s, u32* sh, u32* total) {
-  const u32 t = threadIdx.x;
-  sh[t] = s;
-  __syncthreads();
-  for (u32 o = 1; o < 256; o <<= 1) {
-    const u32 y = t >= o ? sh[t - o] : 0u;
-    __syncthreads();
-    sh[t] += y;
-    __syncthreads();
-  }
-  const u32 incl = sh[t];
-  *total = sh[255];
-  __syncthreads();
-  return incl - s;
-}
-
 // One workgroup per query b: group counts -> exclusive bases in place;
 // K_b -> tot[b].
-__global__ void __launch_bounds__(256) ms_bscan(u32 G, u32* __restrict__ gcnt, u32* __restrict__ tot,
-                                                unsigned long long* bytes) {
-  __shared__ u32 sh[256];
+// One wave per query and no LDS, so the pass also finds room on CUs whose
+// LDS the search kernel holds.
+__global__ void __launch_bounds__(64) ms_bscan(u32 G, u32* __restrict__ gcnt, u32* __restrict__ tot,
+                                               unsigned long long* bytes) {
   const u32 b = blockIdx.x, t = threadIdx.x;
-  const u32 per = (G + 255) / 256;
+  const u32 per = (G + 63) / 64;
   const u32 g0 = min(G, t * per), g1 = min(G, g0 + per);
   u32 s = 0;
   for (u32 g = g0; g < g1; g++) s += gcnt[(u64)g * 64 + b];
-  u32 total;
-  u32 run = ms_block_excl(s, sh, &total);
+  const u32 incl = ms_wave_incl(s);
+  u32 run = incl - s;
   for (u32 g = g0; g < g1; g++) {
     const u32 c = gcnt[(u64)g * 64 + b];
     gcnt[(u64)g * 64 + b] = run;
     run += c;
   }
-  if (t == 0) tot[b] = total;
+  if (t == 63) tot[b] = incl;
   ms_count_bytes(bytes, (u64)(g1 - g0) * 12);
 }
 
@@ -308,127 +345,298 @@ __global__ void __launch_bounds__(256) ms_scatter(u32 n, u32 G, const u64* __res
   ms_count_bytes(bytes, nbytes);
 }
 
-// Local degree of every (query, kept vertex) into loff[id] (scanned later).
-// One wave per kept vertex; lane b counts query b's kept neighbours.
+// One adjacency unit (at most MS_UNIT entries) of a kept vertex: lane i
+// loads entries i, i+64, i+128, i+192 (four loads, then four gathers in
+// flight) and keeps w[t] = the neighbour id and mw[t] = the queries that
+// keep both the vertex (mu) and that neighbour.
+__device__ __forceinline__ void ms_unit_load(const int32_t* __restrict__ adj,
+                                             const u64* __restrict__ vis, u64 mu, i64 b0, u32 len,
+                                             u32 w[4], u64 mw[4], u64& nbytes) {
+  const u32 lane = threadIdx.x & 63;
+#pragma unroll
+  for (int t = 0; t < 4; t++) w[t] = lane + 64u * t < len ? (u32)adj[b0 + lane + 64 * t] : 0u;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    mw[t] = 0;
+    if (lane + 64u * t < len) {
+      mw[t] = vis[w[t]] & mu;
+      nbytes += 12;
+    }
+  }
+}
+
+// Local degree of every (query, kept vertex).  One wave per adjacency unit;
+// lane b counts query b's kept neighbours in the unit as the popcount of
+// column b of the transposed keep masks.  A single-unit vertex writes
+// loff[id] directly; a hub's units write partial counts to hubcnt, which
+// ms_hubsum turns into unit offsets and the degree.
 __global__ void __launch_bounds__(256) ms_degree(const i64* __restrict__ off,
-                                                 const int32_t* __restrict__ adj, u32 n,
+                                                 const int32_t* __restrict__ adj,
+                                                 const MsUnit* __restrict__ units, u32 nunits,
                                                  const u64* __restrict__ vis, u64 valid,
                                                  const u32* __restrict__ locid,
                                                  const MsBatch* __restrict__ B,
+                                                 u32* __restrict__ hubcnt,
                                                  unsigned long long* bytes) {
   const u32 lane = threadIdx.x & 63;
-  const u32 wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const u32 wid = ms_wave_id();
   const u32 nw = (gridDim.x * blockDim.x) >> 6;
   u64 nbytes = 0;
-  for (u32 u = wid; u < n; u += nw) {
-    const u64 mu = vis[u] & valid;
-    if (lane == 0) nbytes += 8;
+  for (u32 k = wid; k < nunits; k += nw) {
+    const MsUnit U = units[k];
+    const u64 mu = vis[U.u] & valid;
+    if (lane == 0) nbytes += 16 + 8;
     if (!mu) continue;
-    const i64 b0 = off[u], e0 = off[u + 1];
+    const i64 b0 = off[U.u] + (i64)U.s * MS_UNIT;
+    const u32 len = (u32)min<i64>(off[U.u + 1] - b0, MS_UNIT);
     if (lane == 0) nbytes += 16;
+    u32 w[4];
+    u64 mw[4];
+    ms_unit_load(adj, vis, mu, b0, len, w, mw, nbytes);
     u32 cnt = 0;
-    for (i64 j = b0; j < e0; j += 64) {
-      u64 mw = 0;
-      if (j + lane < e0) {
-        mw = vis[(u32)adj[j + lane]] & mu;
-        nbytes += 12;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+      if (64u * t < len) cnt += (u32)__popcll(ms_transpose(mw[t]));
+    if (U.h == MS_NOHUB) {
+      if ((mu >> lane) & 1ull) {
+        const u32 id = locid[(u64)U.u * MS_BITS + lane];
+        B->P[lane].loff[id] = cnt;
+        nbytes += 8;
       }
-      u64 any = ms_or(mw);
-      while (any) {
-        const int b = __ffsll((unsigned long long)any) - 1;
-        any &= any - 1;
-        const u64 bal = __ballot((mw >> b) & 1ull);
-        if ((int)lane == b) cnt += (u32)__popcll(bal);
-      }
+    } else {
+      hubcnt[(u64)U.h * 64 + lane] = cnt;
+      nbytes += 4;
     }
+  }
+  ms_count_bytes(bytes, nbytes);
+}
+
+// One wave per hub vertex: lane b turns the hub's per-unit counts into
+// exclusive unit offsets (in place) and writes the degree to loff[id].
+__global__ void __launch_bounds__(256) ms_hubsum(const MsHub* __restrict__ hubs, u32 nhubs,
+                                                 const u64* __restrict__ vis, u64 valid,
+                                                 const u32* __restrict__ locid,
+                                                 const MsBatch* __restrict__ B,
+                                                 u32* __restrict__ hubcnt,
+                                                 unsigned long long* bytes) {
+  const u32 lane = threadIdx.x & 63;
+  const u32 wid = ms_wave_id();
+  const u32 nw = (gridDim.x * blockDim.x) >> 6;
+  u64 nbytes = 0;
+  for (u32 k = wid; k < nhubs; k += nw) {
+    const MsHub H = hubs[k];
+    const u64 mu = vis[H.u] & valid;
+    if (lane == 0) nbytes += 16 + 8;
+    if (!mu) continue;
+    u32 run = 0;
+    for (u32 t = 0; t < H.nu; t++) {
+      u32* p = &hubcnt[(u64)(H.h0 + t) * 64 + lane];
+      const u32 c = *p;
+      *p = run;
+      run += c;
+    }
+    nbytes += (u64)H.nu * 8;
     if ((mu >> lane) & 1ull) {
-      const u32 id = locid[(u64)u * MS_BITS + lane];
-      B->P[lane].loff[id] = cnt;
+      const u32 id = locid[(u64)H.u * MS_BITS + lane];
+      B->P[lane].loff[id] = run;
       nbytes += 8;
     }
   }
   ms_count_bytes(bytes, nbytes);
 }
 
-// One workgroup per query: loff[0..K) exclusive scan in place,
-// loff[K] = nnz -> tot[64 + b].
-__global__ void __launch_bounds__(256) ms_scan_loff(const MsBatch* __restrict__ B,
-                                                    u32* __restrict__ tot, u64 valid,
-                                                    unsigned long long* bytes) {
-  __shared__ u32 sh[256];
-  const u32 b = blockIdx.x, t = threadIdx.x;
-  if (!((valid >> b) & 1ull)) return;
-  u32* a = B->P[b].loff;
+// Exclusive scan of every valid query's loff[0..K) in place, loff[K] = nnz
+// -> tot[64 + b], in three passes with no LDS: chunk sums, chunk bases per
+// query, then each chunk rescanned from its base.  A chunk is MS_SCAN_CHUNK
+// entries, one wave, 64 contiguous entries per lane (16 aligned 16-byte
+// loads: loff starts on an arena granule).  Wave w of the first and last
+// passes takes query w / cmax, chunk w % cmax.
+#define MS_SCAN_CHUNK 4096
+
+__device__ __forceinline__ u32 ms_lane_load(const u32* __restrict__ a, u64 i0, u32 K, uint4 v[16]) {
+  u32 s = 0;
+  if (i0 + 64 <= K) {
+    const uint4* p = (const uint4*)(a + i0);
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = p[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const u64 i = i0 + 4 * k;
+      v[k].x = i < K ? a[i] : 0u;
+      v[k].y = i + 1 < K ? a[i + 1] : 0u;
+      v[k].z = i + 2 < K ? a[i + 2] : 0u;
+      v[k].w = i + 3 < K ? a[i + 3] : 0u;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) s += v[k].x + v[k].y + v[k].z + v[k].w;
+  return s;
+}
+
+__global__ void __launch_bounds__(256) ms_loff_sum(const MsBatch* __restrict__ B,
+                                                   const u32* __restrict__ tot, u64 valid,
+                                                   u32 cmax, u32* __restrict__ csum) {
+  const u32 lane = threadIdx.x & 63;
+  const u32 wid = ms_wave_id();
+  const u32 b = wid / cmax, c = wid % cmax;
+  if (b >= 64 || !((valid >> b) & 1ull)) return;
   const u32 K = tot[b];
-  u32 carry = 0;
-  for (u32 base = 0; base < K; base += 1024) {
-    const u32 i0 = base + t * 4;
-    u32 x[4];
-    u32 s = 0;
-    for (int k = 0; k < 4; k++) {
-      x[k] = i0 + k < K ? a[i0 + k] : 0u;
-      s += x[k];
-    }
-    u32 tile;
-    u32 run = carry + ms_block_excl(s, sh, &tile);
-    for (int k = 0; k < 4; k++) {
-      if (i0 + k < K) a[i0 + k] = run;
-      run += x[k];
-    }
-    carry += tile;
+  const u64 c0 = (u64)c * MS_SCAN_CHUNK;
+  if (c0 >= K) return;
+  const u64 i0 = c0 + 64ull * lane;
+  uint4 v[16];
+  const u32 s = i0 < K ? ms_lane_load(B->P[b].loff, i0, K, v) : 0u;
+  const u32 t = __shfl(ms_wave_incl(s), 63);
+  if (lane == 0) csum[(u64)b * cmax + c] = t;
+}
+
+// One wave per query: chunk sums -> exclusive chunk bases; loff[K] and
+// tot[64 + b] get the total.
+__global__ void __launch_bounds__(64) ms_loff_base(const MsBatch* __restrict__ B,
+                                                   u32* __restrict__ tot, u64 valid, u32 cmax,
+                                                   u32* __restrict__ csum) {
+  const u32 b = blockIdx.x, lane = threadIdx.x;
+  if (!((valid >> b) & 1ull)) return;
+  const u32 K = tot[b];
+  const u32 nc = (K + MS_SCAN_CHUNK - 1) / MS_SCAN_CHUNK;
+  const u32 per = (nc + 63) / 64;
+  const u32 c0 = min(nc, lane * per), c1 = min(nc, c0 + per);
+  u32* cs = csum + (u64)b * cmax;
+  u32 s = 0;
+  for (u32 c = c0; c < c1; c++) s += cs[c];
+  const u32 incl = ms_wave_incl(s);
+  u32 run = incl - s;
+  for (u32 c = c0; c < c1; c++) {
+    const u32 x = cs[c];
+    cs[c] = run;
+    run += x;
   }
-  if (t == 0) {
-    a[K] = carry;
-    tot[64 + b] = carry;
+  if (lane == 63) {
+    B->P[b].loff[K] = incl;
+    tot[64 + b] = incl;
   }
-  ms_count_bytes(bytes, t == 0 ? (u64)K * 8 + 4 : 0);
+}
+
+__global__ void __launch_bounds__(256) ms_loff_apply(const MsBatch* __restrict__ B,
+                                                     const u32* __restrict__ tot, u64 valid,
+                                                     u32 cmax, const u32* __restrict__ csum,
+                                                     unsigned long long* bytes) {
+  const u32 lane = threadIdx.x & 63;
+  const u32 wid = ms_wave_id();
+  const u32 b = wid / cmax, c = wid % cmax;
+  if (b >= 64 || !((valid >> b) & 1ull)) return;
+  const u32 K = tot[b];
+  const u64 c0 = (u64)c * MS_SCAN_CHUNK;
+  if (c0 >= K) return;
+  u32* a = B->P[b].loff;
+  const u64 i0 = c0 + 64ull * lane;
+  uint4 v[16];
+  const u32 s = i0 < K ? ms_lane_load(a, i0, K, v) : 0u;
+  u32 run = csum[(u64)b * cmax + c] + ms_wave_incl(s) - s;
+  u64 nbytes = 0;
+  if (i0 < K) {
+    uint4 o[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      o[k].x = run; run += v[k].x;
+      o[k].y = run; run += v[k].y;
+      o[k].z = run; run += v[k].z;
+      o[k].w = run; run += v[k].w;
+    }
+    if (i0 + 64 <= K) {
+      uint4* p = (uint4*)(a + i0);
+#pragma unroll
+      for (int k = 0; k < 16; k++) p[k] = o[k];
+      nbytes = 64 * 8;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        const u64 i = i0 + 4 * k;
+        if (i < K) a[i] = o[k].x;
+        if (i + 1 < K) a[i + 1] = o[k].y;
+        if (i + 2 < K) a[i + 2] = o[k].z;
+        if (i + 3 < K) a[i + 3] = o[k].w;
+      }
+      nbytes = (K - i0) * 8;
+    }
+  }
+  ms_count_bytes(bytes, nbytes);
 }
 
 // Local adjacency: for every (query, kept vertex u), the local ids of u's
 // kept neighbours in ascending order at ladj[loff[id(u)] ..].
+// One wave per adjacency unit.  Lane b writes query b's list: for each
+// neighbour j of the unit in order (a wave-uniform loop), every lane whose
+// query keeps both ends loads its entry of j's locid row -- one coalesced
+// 256-byte row per neighbour, 16 rows in flight -- and appends it.  Column
+// b of the transposed keep masks says which neighbours query b keeps.
 __global__ void __launch_bounds__(256) ms_fill(const i64* __restrict__ off,
-                                               const int32_t* __restrict__ adj, u32 n,
+                                               const int32_t* __restrict__ adj,
+                                               const MsUnit* __restrict__ units, u32 nunits,
                                                const u64* __restrict__ vis, u64 valid,
                                                const u32* __restrict__ locid,
                                                const MsBatch* __restrict__ B,
+                                               const u32* __restrict__ hubcnt,
                                                unsigned long long* bytes) {
   const u32 lane = threadIdx.x & 63;
-  const u32 wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const u32 wid = ms_wave_id();
   const u32 nw = (gridDim.x * blockDim.x) >> 6;
-  const u64 below = ms_below();
   u64 nbytes = 0;
-  for (u32 u = wid; u < n; u += nw) {
-    const u64 mu = vis[u] & valid;
-    if (lane == 0) nbytes += 8;
+  for (u32 k = wid; k < nunits; k += nw) {
+    const MsUnit U = units[k];
+    const u64 mu = vis[U.u] & valid;
+    if (lane == 0) nbytes += 16 + 8;
     if (!mu) continue;
-    const i64 b0 = off[u], e0 = off[u + 1];
+    const i64 b0 = off[U.u] + (i64)U.s * MS_UNIT;
+    const u32 len = (u32)min<i64>(off[U.u + 1] - b0, MS_UNIT);
     if (lane == 0) nbytes += 16;
-    u32 start = 0;  // lane b: where query b's list of u starts
+    u32 w[4];
+    u64 mw[4];
+    ms_unit_load(adj, vis, mu, b0, len, w, mw, nbytes);
+    // lane b: where query b's entries of this unit go
+    u64 dst = 0;
     if ((mu >> lane) & 1ull) {
-      start = B->P[lane].loff[locid[(u64)u * MS_BITS + lane]];
+      const MsQuery& P = B->P[lane];
+      u32* d = P.ladj + P.loff[locid[(u64)U.u * MS_BITS + lane]];
+      if (U.h != MS_NOHUB) d += hubcnt[(u64)U.h * 64 + lane];
+      dst = (u64)d;
       nbytes += 8;
     }
-    u32 run = 0;  // lane b: entries written so far for query b
-    for (i64 j = b0; j < e0; j += 64) {
-      u64 mw = 0;
-      u32 w = 0;
-      if (j + lane < e0) {
-        w = (u32)adj[j + lane];
-        mw = vis[w] & mu;
-        nbytes += 12;
-      }
-      u64 any = ms_or(mw);
-      while (any) {
-        const int b = __ffsll((unsigned long long)any) - 1;
-        any &= any - 1;
-        const u64 bal = __ballot((mw >> b) & 1ull);
-        const u32 pos0 = __shfl(start + run, b);
-        if ((mw >> b) & 1ull) {
-          B->P[b].ladj[pos0 + (u32)__popcll(bal & below)] = locid[(u64)w * MS_BITS + b];
-          nbytes += 8;
+    // 8 x 8 tiles: lane (ji, qi) = lane / 8, lane % 8 takes query qb*8+qi
+    // and neighbour jb*8+ji of 64-entry block t.  A load instruction reads
+    // eight 32-byte pieces (8 queries of 8 rows) and a store writes eight
+    // runs of up to 8 consecutive entries (8 neighbours of 8 queries), so
+    // both touch 8 lines where a whole-row or whole-column form touches 64.
+    const u32 qi = lane & 7, ji = lane >> 3;
+    __attribute__((address_space(1))) u32* dq[8];  // lane's 8 queries' write positions
+#pragma unroll
+    for (int qb = 0; qb < 8; qb++)
+      dq[qb] = (__attribute__((address_space(1))) u32*)ms_shfl64(dst, qb * 8 + (int)qi);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (64u * t >= len) break;
+      const u64 col = ms_transpose(mw[t]);  // lane b: bit j = query b keeps entry j
+      const u32 lim = min(64u, len - 64u * t);
+      u32 wj[8];  // neighbour jb*8+ji (0 past the unit's end, still a vertex)
+#pragma unroll
+      for (int jb = 0; jb < 8; jb++) wj[jb] = __shfl(w[t], jb * 8 + (int)ji);
+#pragma unroll
+      for (int qb = 0; qb < 8; qb++) {
+        const u64 colq = ms_shfl64(col, qb * 8 + (int)qi);
+        if (!__ballot(colq != 0)) continue;
+        u32 r[8];
+#pragma unroll
+        for (int jb = 0; jb < 8; jb++)
+          r[jb] = 8u * jb < lim ? locid[(u64)wj[jb] * MS_BITS + qb * 8 + qi] : 0u;
+#pragma unroll
+        for (int jb = 0; jb < 8; jb++) {
+          const u32 bits8 = (u32)(colq >> (8 * jb)) & 0xFFu;
+          if ((bits8 >> ji) & 1u) dq[qb][__popc(bits8 & ((1u << ji) - 1u))] = r[jb];
+          dq[qb] += __popc(bits8);
         }
-        if ((int)lane == b) run += (u32)__popcll(bal);
       }
+      nbytes += 8ull * (u32)__popcll(col);
     }
   }
   ms_count_bytes(bytes, nbytes);
@@ -437,6 +645,9 @@ __global__ void __launch_bounds__(256) ms_fill(const i64* __restrict__ off,
 // ---------------------------------------------------------------------------
 // Launchers.  B points to the batch header in device memory.
 // ---------------------------------------------------------------------------
+// Grid-stride kernels launch at most one generation of resident waves
+// (2048 blocks of four waves = 8 waves on each SIMD of 256 CUs).
+#define MS_GRID 2048
 static u32 ms_blocks(u64 threads, u32 cap) {
   u64 b = (threads + 255) / 256;
   if (b == 0) b = 1;
@@ -446,7 +657,7 @@ static u32 ms_blocks(u64 threads, u32 cap) {
 extern "C" hipError_t scs_launch_ms_init(u32 n, const u32* core, const MsBatch* B, u64* elig,
                                          u64* vis, u64* fr, u32* flags, u32 nflags,
                                          unsigned long long* bytes, hipStream_t s) {
-  ms_init<<<ms_blocks(n, 8192), 256, 0, s>>>(n, core, B, elig, vis, fr, flags, nflags, bytes);
+  ms_init<<<ms_blocks(n, MS_GRID), 256, 0, s>>>(n, core, B, elig, vis, fr, flags, nflags, bytes);
   return hipGetLastError();
 }
 extern "C" hipError_t scs_launch_ms_seed(const MsBatch* B, u64* vis, u64* fr, u16* dist16,
@@ -459,43 +670,61 @@ extern "C" hipError_t scs_launch_ms_level(const i64* off, const int32_t* adj, u3
                                           u64* nf, u16* dist16, const u32* flag_prev,
                                           u32* flag_cur, unsigned long long* bytes,
                                           hipStream_t s) {
-  ms_level<<<ms_blocks(((u64)n + 3) / 4 * 64, 16384), 256, 0, s>>>(
+  ms_level<<<ms_blocks(((u64)n + 3) / 4 * 64, MS_GRID), 256, 0, s>>>(
       off, adj, n, level, act, elig, vis, fr, nf, dist16, flag_prev, flag_cur, bytes);
   return hipGetLastError();
 }
 extern "C" hipError_t scs_launch_ms_count(u32 n, u32 G, const u64* vis, u64 valid, u32* gcnt,
                                           unsigned long long* bytes, hipStream_t s) {
-  ms_count<<<ms_blocks((u64)G * 64, 8192), 256, 0, s>>>(n, G, vis, valid, gcnt, bytes);
+  ms_count<<<ms_blocks((u64)G * 64, MS_GRID), 256, 0, s>>>(n, G, vis, valid, gcnt, bytes);
   return hipGetLastError();
 }
 extern "C" hipError_t scs_launch_ms_bscan(u32 nb, u32 G, u32* gcnt, u32* tot,
                                           unsigned long long* bytes, hipStream_t s) {
-  ms_bscan<<<nb, 256, 0, s>>>(G, gcnt, tot, bytes);
+  ms_bscan<<<nb, 64, 0, s>>>(G, gcnt, tot, bytes);
   return hipGetLastError();
 }
 extern "C" hipError_t scs_launch_ms_scatter(u32 n, u32 G, const u64* vis, u64 valid,
                                             const u32* gcnt, const u32* core, const u16* dist16,
                                             const MsBatch* B, u32* locid, u32* qloc,
                                             unsigned long long* bytes, hipStream_t s) {
-  ms_scatter<<<ms_blocks((u64)G * 64, 8192), 256, 0, s>>>(n, G, vis, valid, gcnt, core, dist16, B,
+  ms_scatter<<<ms_blocks((u64)G * 64, MS_GRID), 256, 0, s>>>(n, G, vis, valid, gcnt, core, dist16, B,
                                                           locid, qloc, bytes);
   return hipGetLastError();
 }
-extern "C" hipError_t scs_launch_ms_degree(const i64* off, const int32_t* adj, u32 n,
+// Local degrees, then the hubs' unit offsets (nhubs may be 0).
+extern "C" hipError_t scs_launch_ms_degree(const i64* off, const int32_t* adj, const MsUnit* units,
+                                           u32 nunits, const MsHub* hubs, u32 nhubs,
                                            const u64* vis, u64 valid, const u32* locid,
-                                           const MsBatch* B, unsigned long long* bytes,
-                                           hipStream_t s) {
-  ms_degree<<<ms_blocks((u64)n * 64, 16384), 256, 0, s>>>(off, adj, n, vis, valid, locid, B, bytes);
+                                           const MsBatch* B, u32* hubcnt,
+                                           unsigned long long* bytes, hipStream_t s) {
+  ms_degree<<<ms_blocks((u64)nunits * 64, MS_GRID), 256, 0, s>>>(off, adj, units, nunits, vis, valid,
+                                                               locid, B, hubcnt, bytes);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nhubs == 0) return e;
+  ms_hubsum<<<ms_blocks((u64)nhubs * 64, MS_GRID), 256, 0, s>>>(hubs, nhubs, vis, valid, locid, B,
+                                                             hubcnt, bytes);
   return hipGetLastError();
 }
+// cmax = chunks of the largest valid K (>= 1); csum holds nb * cmax words.
 extern "C" hipError_t scs_launch_ms_scan_loff(u32 nb, const MsBatch* B, u32* tot, u64 valid,
-                                              unsigned long long* bytes, hipStream_t s) {
-  ms_scan_loff<<<nb, 256, 0, s>>>(B, tot, valid, bytes);
+                                              u32 cmax, u32* csum, unsigned long long* bytes,
+                                              hipStream_t s) {
+  const u32 blocks = (u32)(((u64)nb * cmax + 3) / 4);
+  ms_loff_sum<<<blocks, 256, 0, s>>>(B, tot, valid, cmax, csum);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  ms_loff_base<<<nb, 64, 0, s>>>(B, tot, valid, cmax, csum);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  ms_loff_apply<<<blocks, 256, 0, s>>>(B, tot, valid, cmax, csum, bytes);
   return hipGetLastError();
 }
-extern "C" hipError_t scs_launch_ms_fill(const i64* off, const int32_t* adj, u32 n, const u64* vis,
-                                         u64 valid, const u32* locid, const MsBatch* B,
+extern "C" hipError_t scs_launch_ms_fill(const i64* off, const int32_t* adj, const MsUnit* units,
+                                         u32 nunits, const u64* vis, u64 valid, const u32* locid,
+                                         const MsBatch* B, const u32* hubcnt,
                                          unsigned long long* bytes, hipStream_t s) {
-  ms_fill<<<ms_blocks((u64)n * 64, 16384), 256, 0, s>>>(off, adj, n, vis, valid, locid, B, bytes);
+  ms_fill<<<ms_blocks((u64)nunits * 64, MS_GRID), 256, 0, s>>>(off, adj, units, nunits, vis, valid,
+                                                             locid, B, hubcnt, bytes);
   return hipGetLastError();
 }
