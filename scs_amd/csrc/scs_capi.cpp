@@ -30,8 +30,11 @@ hipError_t scs_launch_core_scan(u32 n, u32 k, const u32* deg, u32* core, u32* F,
 hipError_t scs_launch_core_peel(const i64* off, const int32_t* adj, u32 k, const u32* F, u32 fn,
                                 u32* deg, u32* core, u32* Fn, u32* fncount, hipStream_t s);
 hipError_t scs_launch_ms_init(u32 n, const u32* core, const MsBatch* B, u64* elig, u64* vis,
-                              u64* fr, u32* flags, u32 nflags, unsigned long long* bytes,
+                              u64* fr, u64* nf, u32* flags, u32 nflags, unsigned long long* bytes,
                               hipStream_t s);
+hipError_t scs_launch_ms_push0(const i64* off, const int32_t* adj, u32 nb, const MsBatch* B,
+                               u64 act, const u64* elig, u64* vis, u64* nf, u16* dist16,
+                               u32* flag0, unsigned long long* bytes, hipStream_t s);
 hipError_t scs_launch_ms_seed(const MsBatch* B, u64* vis, u64* fr, u16* dist16, hipStream_t s);
 hipError_t scs_launch_ms_level(const i64* off, const int32_t* adj, u32 n, u32 level, u64 act,
                                const u64* elig, u64* vis, const u64* fr, u64* nf, u16* dist16,
@@ -44,13 +47,13 @@ hipError_t scs_launch_ms_bscan(u32 nb, u32 G, u32* gcnt, u32* tot, unsigned long
 hipError_t scs_launch_ms_scatter(u32 n, u32 G, const u64* vis, u64 valid, const u32* gcnt,
                                  const u32* core, const u16* dist16, const MsBatch* B, u32* locid,
                                  u32* qloc, unsigned long long* bytes, hipStream_t s);
-hipError_t scs_launch_ms_degree(const i64* off, const int32_t* adj, const MsUnit* units,
-                                u32 nunits, const MsHub* hubs, u32 nhubs, const u64* vis,
-                                u64 valid, const u32* locid, const MsBatch* B, u32* hubcnt,
+hipError_t scs_launch_ms_degree(const int32_t* adj, const MsUnit* units, u32 nunits,
+                                const MsHub* hubs, u32 nhubs, const u64* vis, u64 valid,
+                                const u32* locid, const MsBatch* B, u32* hubcnt,
                                 unsigned long long* bytes, hipStream_t s);
 hipError_t scs_launch_ms_scan_loff(u32 nb, const MsBatch* B, u32* tot, u64 valid, u32 cmax,
                                    u32* csum, unsigned long long* bytes, hipStream_t s);
-hipError_t scs_launch_ms_fill(const i64* off, const int32_t* adj, const MsUnit* units, u32 nunits,
+hipError_t scs_launch_ms_fill(const int32_t* adj, const MsUnit* units, u32 nunits,
                               const u64* vis, u64 valid, const u32* locid, const MsBatch* B,
                               const u32* hubcnt, unsigned long long* bytes, hipStream_t s);
 hipError_t scs_launch_epoch(const EngineArgs* A, hipStream_t s);
@@ -461,12 +464,16 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
     for (u32 v = 0; v < n; v++) {
       const u64 deg = (u64)(E->g->hg.off[v + 1] - E->g->hg.off[v]);
       const u64 nu = (deg + MS_UNIT - 1) / MS_UNIT;
+      const u64 b0 = (u64)E->g->hg.off[v];
       if (nu <= 1) {
-        units.push_back(MsUnit{v, 0, MS_NOHUB, 0});
+        units.push_back(MsUnit{v, MS_NOHUB, b0 | (deg << 48)});
         continue;
       }
       hubs.push_back(MsHub{v, hub_units, (u32)nu, 0});
-      for (u32 si = 0; si < nu; si++) units.push_back(MsUnit{v, si, hub_units + si, 0});
+      for (u32 si = 0; si < nu; si++) {
+        const u64 ulen = std::min<u64>(MS_UNIT, deg - (u64)si * MS_UNIT);
+        units.push_back(MsUnit{v, hub_units + si, (b0 + (u64)si * MS_UNIT) | (ulen << 48)});
+      }
       hub_units += (u32)nu;
     }
     E->nunits = (u32)units.size();
@@ -724,13 +731,23 @@ static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32*
   const int32_t* adj = E->g->d_adj;
   HIPCHK(hipEventRecord(E->pe0, s));
   HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
-  HIPCHK(scs_launch_ms_init(n, E->g->d_core, E->d_batch, E->d_elig, E->d_vis, E->d_fr, E->d_flags,
-                            SCS_MS_LEVELS, E->d_prepbytes, s));
+  HIPCHK(scs_launch_ms_init(n, E->g->d_core, E->d_batch, E->d_elig, E->d_vis, E->d_fr, E->d_nf,
+                            E->d_flags, SCS_MS_LEVELS, E->d_prepbytes, s));
   HIPCHK(scs_launch_ms_seed(E->d_batch, E->d_vis, E->d_fr, E->d_dist16, s));
   // BFS levels; a level that finds nothing ends the later ones on the
   // device, and the host checks every 8 levels
   u64 *fr = E->d_fr, *nf = E->d_nf;
   u32 L = 0;
+  if (maxU > 0) {
+    // level 0 pushes from the query vertices (top-down)
+    u64 act = 0;
+    for (u32 b = 0; b < cnt; b++)
+      if (ubd[b] > 0) act |= 1ull << b;
+    HIPCHK(scs_launch_ms_push0(off, adj, cnt, E->d_batch, act, E->d_elig, E->d_vis, nf,
+                               E->d_dist16, E->d_flags, E->d_prepbytes, s));
+    std::swap(fr, nf);
+    L = 1;
+  }
   while (L < maxU) {
     const u32 stop = std::min(maxU, L + 8);
     for (; L < stop; L++) {
@@ -784,7 +801,7 @@ static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32*
     HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
     HIPCHK(scs_launch_ms_scatter(n, G, E->d_vis, valid, E->d_gcnt, E->g->d_core, E->d_dist16,
                                  E->d_batch, E->d_locid64, E->d_tot + 128, E->d_prepbytes, s));
-    HIPCHK(scs_launch_ms_degree(off, adj, E->d_units, E->nunits, E->d_hubs, E->nhubs, E->d_vis,
+    HIPCHK(scs_launch_ms_degree(adj, E->d_units, E->nunits, E->d_hubs, E->nhubs, E->d_vis,
                                 valid, E->d_locid64, E->d_batch, E->d_hubcnt, E->d_prepbytes, s));
     HIPCHK(scs_launch_ms_scan_loff(cnt, E->d_batch, E->d_tot, valid,
                                    std::max<u32>(1, (maxK + 4095) / 4096), E->d_csum,
@@ -811,7 +828,7 @@ static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32*
     }
     if (valid) {
       HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
-      HIPCHK(scs_launch_ms_fill(off, adj, E->d_units, E->nunits, E->d_vis, valid, E->d_locid64,
+      HIPCHK(scs_launch_ms_fill(adj, E->d_units, E->nunits, E->d_vis, valid, E->d_locid64,
                                 E->d_batch, E->d_hubcnt, E->d_prepbytes, s));
     }
   }

@@ -168,10 +168,9 @@ struct MsBatch {
 #define MS_UNIT 256
 #define MS_NOHUB 0xFFFFFFFFu
 struct MsUnit {
-  u32 u;  // vertex
-  u32 s;  // unit index within the vertex: entries off[u] + s*MS_UNIT ..
-  u32 h;  // hub unit number, or MS_NOHUB
-  u32 pad;
+  u32 u;    // vertex
+  u32 h;    // hub unit number, or MS_NOHUB
+  u64 pos;  // first adjacency entry (bits 0-47) | entries, at most MS_UNIT (bits 48-63)
 };
 struct MsHub {
   u32 u;

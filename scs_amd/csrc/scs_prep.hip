@@ -12,15 +12,17 @@
 //            the first cnt(v) bits: one compare per query, no table.
 //   vis[v]   queries whose BFS reached v (= v is kept for them).
 //   fr / nf  queries for which v is in the current / next frontier.
-// A level is bottom-up (pull): every vertex that some active query still
-// wants ORs the frontier masks of its neighbours, stopping once it has every
-// bit it wants.  No atomics, and the adjacency of a vertex is read once per
-// level for all 64 queries.  Kept vertex v of query b gets local id
+// Level 0 pushes from the query vertices (top-down).  Every later level is
+// bottom-up (pull): every vertex that some active query still wants ORs the
+// frontier masks of its neighbours, stopping once it has every bit it
+// wants.  No atomics, and the adjacency of a vertex is read once per level
+// for all 64 queries.  Kept vertex v of query b gets local id
 //   base[g][b] + rank of v among group g's vertices kept by b
 // (g = v / 64, ranks by ballot), so local ids ascend with original ids as
 // the reference's compaction order does.  The local degree and adjacency
 // passes read the adjacency of each kept vertex once for every query of
-// the batch that kept it, with one ballot per query per 64-entry chunk.
+// the batch that kept it; a 64 x 64 bit transpose of the per-neighbour keep
+// masks gives every query its own bits.
 //
 // Outputs per query b (device pointers in MsQuery): loff[K+1] (exclusive
 // scan of local degrees, loff[K] = nnz), ldist[K], lcore[K], lorig[K] and
@@ -108,8 +110,9 @@ __device__ __forceinline__ u32 ms_wave_id() {
 __global__ void __launch_bounds__(256) ms_init(u32 n, const u32* __restrict__ core,
                                                const MsBatch* __restrict__ B,
                                                u64* __restrict__ elig, u64* __restrict__ vis,
-                                               u64* __restrict__ fr, u32* __restrict__ flags,
-                                               u32 nflags, unsigned long long* bytes) {
+                                               u64* __restrict__ fr, u64* __restrict__ nf,
+                                               u32* __restrict__ flags, u32 nflags,
+                                               unsigned long long* bytes) {
   const u32 tid = blockIdx.x * blockDim.x + threadIdx.x;
   const u32 nt = gridDim.x * blockDim.x;
   u64 nb = 0;
@@ -121,7 +124,8 @@ __global__ void __launch_bounds__(256) ms_init(u32 n, const u32* __restrict__ co
     elig[v] = cnt >= 64 ? ~0ull : ((1ull << cnt) - 1ull);
     vis[v] = 0;
     fr[v] = 0;
-    nb += 4 + 24;
+    nf[v] = 0;
+    nb += 4 + 32;
   }
   for (u32 i = tid; i < nflags; i += nt) flags[i] = 0;
   ms_count_bytes(bytes, nb);
@@ -139,11 +143,47 @@ __global__ void ms_seed(const MsBatch* __restrict__ B, u64* __restrict__ vis,
   }
 }
 
+// Level 0, top-down: the frontier is the query vertices themselves, so
+// rather than every eligible vertex scanning its adjacency for them (the
+// bottom-up form of the later levels), the wave of query b pushes bit b to
+// the eligible neighbours of q_b.  The atomicOr on vis decides which wave
+// reached a vertex first for a bit, so each (vertex, query) is taken once.
+// Sets flags[0] if anything was reached.
+__global__ void __launch_bounds__(64) ms_push0(const i64* __restrict__ off,
+                                               const int32_t* __restrict__ adj,
+                                               const MsBatch* __restrict__ B, u64 act,
+                                               const u64* __restrict__ elig, u64* __restrict__ vis,
+                                               u64* __restrict__ nf, u16* __restrict__ dist16,
+                                               u32* __restrict__ flag0, unsigned long long* bytes) {
+  const u32 b = blockIdx.x, lane = threadIdx.x;
+  if (!((act >> b) & 1ull)) return;
+  const u32 q = B->q[b];
+  const u64 bit = 1ull << b;
+  const i64 b0 = off[q], e0 = off[q + 1];
+  bool found = false;
+  u64 nbytes = lane == 0 ? 20 : 0;
+  for (i64 j = b0 + lane; j < e0; j += 64) {
+    const u32 w = (u32)adj[j];
+    nbytes += 12;
+    if (elig[w] & bit) {
+      const u64 old = atomicOr((unsigned long long*)&vis[w], (unsigned long long)bit);
+      if (!(old & bit)) {
+        atomicOr((unsigned long long*)&nf[w], (unsigned long long)bit);
+        dist16[(u64)w * MS_BITS + b] = 1;
+        found = true;
+        nbytes += 18;
+      }
+    }
+  }
+  if (__ballot(found) && lane == 0) atomicOr(flag0, 1u);
+  ms_count_bytes(bytes, nbytes);
+}
+
 // One BFS level for every query whose depth bound exceeds `level` (act).
 // Runs only if the previous level found something (flag_prev).  A 16-lane
-// segment takes one vertex (four per wave) and reads 16 adjacency entries
+// segment takes one vertex (four per wave) and reads 64 adjacency entries
 // per step, stopping once it holds every bit it wants; vertices of degree
-// above MS_HUB are taken by the whole wave.
+// above MS_HUB are taken by the whole wave, 512 entries per step.
 #define MS_HUB 512
 
 __device__ __forceinline__ u64 ms_seg_or(u64 x) {
@@ -165,12 +205,15 @@ __global__ void __launch_bounds__(256) ms_level(const i64* __restrict__ off,
                                                 unsigned long long* bytes) {
   if (level > 0 && *flag_prev == 0) return;
   const u32 lane = threadIdx.x & 63, seg = lane >> 4, sl = lane & 15;
-  const u64 wid = (blockIdx.x * (u64)blockDim.x + threadIdx.x) >> 6;
+  const u64 wid = ms_wave_id();
   const u64 nw = ((u64)gridDim.x * blockDim.x) >> 6;
   u64 nbytes = 0;
   bool found = false;
-  for (u64 base = wid * 4; base < n; base += nw * 4) {
-    const u64 vv = base + seg;
+  // segment s of wave w takes vertex (4*it + s)*nw + w at step it:
+  // neighbouring ids go to different waves, so a run of hubs (generators
+  // often number them first) is spread over the grid, not queued in one wave
+  for (u64 it = 0; it * 4 * nw < n; it++) {
+    const u64 vv = (4 * it + seg) * nw + wid;
     const bool in = vv < n;
     const u32 v = (u32)vv;
     u64 want = 0, visv = 0;
@@ -210,7 +253,7 @@ __global__ void __launch_bounds__(256) ms_level(const i64* __restrict__ off,
       j += 64;
       if (active && (acc == want || j - (i64)sl >= e0)) active = false;
     }
-    // hubs: the whole wave, 256 entries per step
+    // hubs: the whole wave, 512 entries per step
     u64 bm = __ballot(big && sl == 0);
     while (bm) {
       const int src = __ffsll((unsigned long long)bm) - 1;
@@ -218,13 +261,13 @@ __global__ void __launch_bounds__(256) ms_level(const i64* __restrict__ off,
       const u64 bwant = ms_shfl64(want, src);
       const i64 bb = (i64)ms_shfl64((u64)b0, src), be = (i64)ms_shfl64((u64)e0, src);
       u64 bacc = 0;
-      for (i64 jj = bb + lane; jj - (i64)lane < be; jj += 256) {
-        u32 w[4];
+      for (i64 jj = bb + lane; jj - (i64)lane < be; jj += 512) {
+        u32 w[8];
 #pragma unroll
-        for (int t = 0; t < 4; t++) w[t] = jj + 64 * t < be ? (u32)adj[jj + 64 * t] : 0u;
+        for (int t = 0; t < 8; t++) w[t] = jj + 64 * t < be ? (u32)adj[jj + 64 * t] : 0u;
         u64 x = 0;
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int t = 0; t < 8; t++)
           if (jj + 64 * t < be) {
             x |= fr[w[t]];
             nbytes += 12;
@@ -345,12 +388,12 @@ __global__ void __launch_bounds__(256) ms_scatter(u32 n, u32 G, const u64* __res
   ms_count_bytes(bytes, nbytes);
 }
 
-// One adjacency unit (at most MS_UNIT entries) of a kept vertex: lane i
-// loads entries i, i+64, i+128, i+192 (four loads, then four gathers in
-// flight) and keeps w[t] = the neighbour id and mw[t] = the queries that
-// keep both the vertex (mu) and that neighbour.
+// One adjacency unit (at most MS_UNIT entries): lane i loads entries i,
+// i+64, i+128, i+192 (four loads, then four gathers in flight) and keeps
+// w[t] = the neighbour id (0 past the unit's end) and mw[t] = the queries
+// that keep that neighbour.
 __device__ __forceinline__ void ms_unit_load(const int32_t* __restrict__ adj,
-                                             const u64* __restrict__ vis, u64 mu, i64 b0, u32 len,
+                                             const u64* __restrict__ vis, i64 b0, u32 len,
                                              u32 w[4], u64 mw[4], u64& nbytes) {
   const u32 lane = threadIdx.x & 63;
 #pragma unroll
@@ -359,19 +402,23 @@ __device__ __forceinline__ void ms_unit_load(const int32_t* __restrict__ adj,
   for (int t = 0; t < 4; t++) {
     mw[t] = 0;
     if (lane + 64u * t < len) {
-      mw[t] = vis[w[t]] & mu;
+      mw[t] = vis[w[t]];
       nbytes += 12;
     }
   }
 }
+__device__ __forceinline__ i64 ms_unit_b0(const MsUnit& U) {
+  return (i64)(U.pos & 0xFFFFFFFFFFFFull);
+}
+__device__ __forceinline__ u32 ms_unit_len(const MsUnit& U) { return (u32)(U.pos >> 48); }
 
 // Local degree of every (query, kept vertex).  One wave per adjacency unit;
 // lane b counts query b's kept neighbours in the unit as the popcount of
 // column b of the transposed keep masks.  A single-unit vertex writes
 // loff[id] directly; a hub's units write partial counts to hubcnt, which
-// ms_hubsum turns into unit offsets and the degree.
-__global__ void __launch_bounds__(256) ms_degree(const i64* __restrict__ off,
-                                                 const int32_t* __restrict__ adj,
+// ms_hubsum turns into unit offsets and the degree.  The unit's vis word,
+// locid row and adjacency are loaded together, before the keep test.
+__global__ void __launch_bounds__(256) ms_degree(const int32_t* __restrict__ adj,
                                                  const MsUnit* __restrict__ units, u32 nunits,
                                                  const u64* __restrict__ vis, u64 valid,
                                                  const u32* __restrict__ locid,
@@ -381,26 +428,26 @@ __global__ void __launch_bounds__(256) ms_degree(const i64* __restrict__ off,
   const u32 lane = threadIdx.x & 63;
   const u32 wid = ms_wave_id();
   const u32 nw = (gridDim.x * blockDim.x) >> 6;
+  u32* const loff = B->P[lane].loff;
   u64 nbytes = 0;
   for (u32 k = wid; k < nunits; k += nw) {
     const MsUnit U = units[k];
+    const i64 b0 = ms_unit_b0(U);
+    const u32 len = ms_unit_len(U);
     const u64 mu = vis[U.u] & valid;
+    const u32 id = locid[(u64)U.u * MS_BITS + lane];
+    u32 w[4] = {0u, 0u, 0u, 0u};
+    u64 mw[4] = {0ull, 0ull, 0ull, 0ull};
+    if (len) ms_unit_load(adj, vis, b0, len, w, mw, nbytes);
     if (lane == 0) nbytes += 16 + 8;
     if (!mu) continue;
-    const i64 b0 = off[U.u] + (i64)U.s * MS_UNIT;
-    const u32 len = (u32)min<i64>(off[U.u + 1] - b0, MS_UNIT);
-    if (lane == 0) nbytes += 16;
-    u32 w[4];
-    u64 mw[4];
-    ms_unit_load(adj, vis, mu, b0, len, w, mw, nbytes);
     u32 cnt = 0;
 #pragma unroll
     for (int t = 0; t < 4; t++)
-      if (64u * t < len) cnt += (u32)__popcll(ms_transpose(mw[t]));
+      if (64u * t < len) cnt += (u32)__popcll(ms_transpose(mw[t] & mu));
     if (U.h == MS_NOHUB) {
       if ((mu >> lane) & 1ull) {
-        const u32 id = locid[(u64)U.u * MS_BITS + lane];
-        B->P[lane].loff[id] = cnt;
+        loff[id] = cnt;
         nbytes += 8;
       }
     } else {
@@ -566,77 +613,95 @@ __global__ void __launch_bounds__(256) ms_loff_apply(const MsBatch* __restrict__
 
 // Local adjacency: for every (query, kept vertex u), the local ids of u's
 // kept neighbours in ascending order at ladj[loff[id(u)] ..].
-// One wave per adjacency unit.  Lane b writes query b's list: for each
-// neighbour j of the unit in order (a wave-uniform loop), every lane whose
-// query keeps both ends loads its entry of j's locid row -- one coalesced
-// 256-byte row per neighbour, 16 rows in flight -- and appends it.  Column
-// b of the transposed keep masks says which neighbours query b keeps.
-__global__ void __launch_bounds__(256) ms_fill(const i64* __restrict__ off,
-                                               const int32_t* __restrict__ adj,
+// One wave per adjacency unit.  Query b's entries of the unit go to its
+// write position dst_b, in neighbour order; column b of the transposed keep
+// masks says which neighbours query b keeps.  The work is cut into tiles of
+// MS_FILL_QT queries x (64 / MS_FILL_QT) neighbours: lane (ji, qi) =
+// (lane / QT, lane % QT) takes query qb*QT+qi and neighbour jb*JT+ji.  A
+// store instruction then writes QT runs of up to JT consecutive entries and
+// a load instruction reads JT pieces of QT entries of the locid rows, where
+// a whole-row or whole-column form touches 64 lines with one of the two.
+// On C2 (Chung-Lu, 100K vertices) 8 x 8 tiles with 32 row loads in flight
+// were fastest; 4 x 16 and 2 x 32 were 1 % and 22 % slower.
+#ifndef MS_FILL_QT
+#define MS_FILL_QT 8
+#endif
+#ifndef MS_FILL_LOADS
+#define MS_FILL_LOADS 32  // row loads in flight per wave
+#endif
+__global__ void __launch_bounds__(256) ms_fill(const int32_t* __restrict__ adj,
                                                const MsUnit* __restrict__ units, u32 nunits,
                                                const u64* __restrict__ vis, u64 valid,
                                                const u32* __restrict__ locid,
                                                const MsBatch* __restrict__ B,
                                                const u32* __restrict__ hubcnt,
                                                unsigned long long* bytes) {
+  constexpr u32 QT = MS_FILL_QT, JT = 64 / QT, QG = MS_FILL_LOADS / QT;
+  constexpr u64 GMASK = QG * QT == 64 ? ~0ull : ((1ull << (QG * QT)) - 1ull);
+  constexpr u64 JMASK = JT == 64 ? ~0ull : ((1ull << JT) - 1ull);
   const u32 lane = threadIdx.x & 63;
   const u32 wid = ms_wave_id();
   const u32 nw = (gridDim.x * blockDim.x) >> 6;
+  const u32 qi = lane % QT, ji = lane / QT;
+  const u64 below = ji ? (~0ull >> (64 - ji)) : 0ull;
+  const u32* const loff = B->P[lane].loff;
+  u32* const ladj = B->P[lane].ladj;
   u64 nbytes = 0;
   for (u32 k = wid; k < nunits; k += nw) {
     const MsUnit U = units[k];
+    const i64 b0 = ms_unit_b0(U);
+    const u32 len = ms_unit_len(U);
     const u64 mu = vis[U.u] & valid;
+    const u32 id = locid[(u64)U.u * MS_BITS + lane];
+    const u32 part = hubcnt[(U.h != MS_NOHUB ? (u64)U.h * 64 : 0ull) + lane] &
+                     (U.h != MS_NOHUB ? ~0u : 0u);
+    u32 w[4] = {0u, 0u, 0u, 0u};
+    u64 mw[4] = {0ull, 0ull, 0ull, 0ull};
+    if (len) ms_unit_load(adj, vis, b0, len, w, mw, nbytes);
     if (lane == 0) nbytes += 16 + 8;
     if (!mu) continue;
-    const i64 b0 = off[U.u] + (i64)U.s * MS_UNIT;
-    const u32 len = (u32)min<i64>(off[U.u + 1] - b0, MS_UNIT);
-    if (lane == 0) nbytes += 16;
-    u32 w[4];
-    u64 mw[4];
-    ms_unit_load(adj, vis, mu, b0, len, w, mw, nbytes);
     // lane b: where query b's entries of this unit go
     u64 dst = 0;
     if ((mu >> lane) & 1ull) {
-      const MsQuery& P = B->P[lane];
-      u32* d = P.ladj + P.loff[locid[(u64)U.u * MS_BITS + lane]];
-      if (U.h != MS_NOHUB) d += hubcnt[(u64)U.h * 64 + lane];
-      dst = (u64)d;
+      dst = (u64)(ladj + loff[id] + part);
       nbytes += 8;
     }
-    // 8 x 8 tiles: lane (ji, qi) = lane / 8, lane % 8 takes query qb*8+qi
-    // and neighbour jb*8+ji of 64-entry block t.  A load instruction reads
-    // eight 32-byte pieces (8 queries of 8 rows) and a store writes eight
-    // runs of up to 8 consecutive entries (8 neighbours of 8 queries), so
-    // both touch 8 lines where a whole-row or whole-column form touches 64.
-    const u32 qi = lane & 7, ji = lane >> 3;
-    __attribute__((address_space(1))) u32* dq[8];  // lane's 8 queries' write positions
+    const u32 nt = (len + 63) / 64;
+    u64 col[4];  // lane b: bit j = query b keeps entry 64t + j
 #pragma unroll
-    for (int qb = 0; qb < 8; qb++)
-      dq[qb] = (__attribute__((address_space(1))) u32*)ms_shfl64(dst, qb * 8 + (int)qi);
+    for (int t = 0; t < 4; t++) col[t] = (u32)t < nt ? ms_transpose(mw[t] & mu) : 0ull;
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      if (64u * t >= len) break;
-      const u64 col = ms_transpose(mw[t]);  // lane b: bit j = query b keeps entry j
-      const u32 lim = min(64u, len - 64u * t);
-      u32 wj[8];  // neighbour jb*8+ji (0 past the unit's end, still a vertex)
+    for (int t = 0; t < 4; t++) nbytes += 8ull * (u32)__popcll(col[t]);
+    // QG query groups at a time, so QG * QT row loads are in flight
+    for (u32 qb0 = 0; qb0 < 64 / QT; qb0 += QG) {
+      if (!((mu >> (qb0 * QT)) & GMASK)) continue;  // none of these queries keeps U.u
+      __attribute__((address_space(1))) u32* dq[QG];
 #pragma unroll
-      for (int jb = 0; jb < 8; jb++) wj[jb] = __shfl(w[t], jb * 8 + (int)ji);
+      for (u32 x = 0; x < QG; x++)
+        dq[x] = (__attribute__((address_space(1))) u32*)ms_shfl64(dst, (int)((qb0 + x) * QT + qi));
 #pragma unroll
-      for (int qb = 0; qb < 8; qb++) {
-        const u64 colq = ms_shfl64(col, qb * 8 + (int)qi);
-        if (!__ballot(colq != 0)) continue;
-        u32 r[8];
+      for (int t = 0; t < 4; t++) {
+        if ((u32)t >= nt) break;
+        u32 wj[QT];  // w[t] is 0 past the unit's end, so every row index is a vertex
 #pragma unroll
-        for (int jb = 0; jb < 8; jb++)
-          r[jb] = 8u * jb < lim ? locid[(u64)wj[jb] * MS_BITS + qb * 8 + qi] : 0u;
+        for (u32 jb = 0; jb < QT; jb++) wj[jb] = (u32)__shfl(w[t], (int)(jb * JT + ji));
+        u32 r[QG][QT];
 #pragma unroll
-        for (int jb = 0; jb < 8; jb++) {
-          const u32 bits8 = (u32)(colq >> (8 * jb)) & 0xFFu;
-          if ((bits8 >> ji) & 1u) dq[qb][__popc(bits8 & ((1u << ji) - 1u))] = r[jb];
-          dq[qb] += __popc(bits8);
+        for (u32 x = 0; x < QG; x++)
+#pragma unroll
+          for (u32 jb = 0; jb < QT; jb++)
+            r[x][jb] = locid[(u64)wj[jb] * MS_BITS + (qb0 + x) * QT + qi];
+#pragma unroll
+        for (u32 x = 0; x < QG; x++) {
+          const u64 colq = ms_shfl64(col[t], (int)((qb0 + x) * QT + qi));
+#pragma unroll
+          for (u32 jb = 0; jb < QT; jb++) {
+            const u64 bits = (colq >> (jb * JT)) & JMASK;
+            if ((bits >> ji) & 1ull) dq[x][__popcll(bits & below)] = r[x][jb];
+            dq[x] += __popcll(bits);
+          }
         }
       }
-      nbytes += 8ull * (u32)__popcll(col);
     }
   }
   ms_count_bytes(bytes, nbytes);
@@ -655,9 +720,17 @@ static u32 ms_blocks(u64 threads, u32 cap) {
 }
 
 extern "C" hipError_t scs_launch_ms_init(u32 n, const u32* core, const MsBatch* B, u64* elig,
-                                         u64* vis, u64* fr, u32* flags, u32 nflags,
+                                         u64* vis, u64* fr, u64* nf, u32* flags, u32 nflags,
                                          unsigned long long* bytes, hipStream_t s) {
-  ms_init<<<ms_blocks(n, MS_GRID), 256, 0, s>>>(n, core, B, elig, vis, fr, flags, nflags, bytes);
+  ms_init<<<ms_blocks(n, MS_GRID), 256, 0, s>>>(n, core, B, elig, vis, fr, nf, flags, nflags,
+                                                bytes);
+  return hipGetLastError();
+}
+extern "C" hipError_t scs_launch_ms_push0(const i64* off, const int32_t* adj, u32 nb,
+                                          const MsBatch* B, u64 act, const u64* elig, u64* vis,
+                                          u64* nf, u16* dist16, u32* flag0,
+                                          unsigned long long* bytes, hipStream_t s) {
+  ms_push0<<<nb, 64, 0, s>>>(off, adj, B, act, elig, vis, nf, dist16, flag0, bytes);
   return hipGetLastError();
 }
 extern "C" hipError_t scs_launch_ms_seed(const MsBatch* B, u64* vis, u64* fr, u16* dist16,
@@ -693,13 +766,13 @@ extern "C" hipError_t scs_launch_ms_scatter(u32 n, u32 G, const u64* vis, u64 va
   return hipGetLastError();
 }
 // Local degrees, then the hubs' unit offsets (nhubs may be 0).
-extern "C" hipError_t scs_launch_ms_degree(const i64* off, const int32_t* adj, const MsUnit* units,
-                                           u32 nunits, const MsHub* hubs, u32 nhubs,
-                                           const u64* vis, u64 valid, const u32* locid,
-                                           const MsBatch* B, u32* hubcnt,
-                                           unsigned long long* bytes, hipStream_t s) {
-  ms_degree<<<ms_blocks((u64)nunits * 64, MS_GRID), 256, 0, s>>>(off, adj, units, nunits, vis, valid,
-                                                               locid, B, hubcnt, bytes);
+extern "C" hipError_t scs_launch_ms_degree(const int32_t* adj, const MsUnit* units, u32 nunits,
+                                           const MsHub* hubs, u32 nhubs, const u64* vis,
+                                           u64 valid, const u32* locid, const MsBatch* B,
+                                           u32* hubcnt, unsigned long long* bytes,
+                                           hipStream_t s) {
+  ms_degree<<<ms_blocks((u64)nunits * 64, MS_GRID), 256, 0, s>>>(adj, units, nunits, vis, valid,
+                                                                locid, B, hubcnt, bytes);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || nhubs == 0) return e;
   ms_hubsum<<<ms_blocks((u64)nhubs * 64, MS_GRID), 256, 0, s>>>(hubs, nhubs, vis, valid, locid, B,
@@ -720,11 +793,11 @@ extern "C" hipError_t scs_launch_ms_scan_loff(u32 nb, const MsBatch* B, u32* tot
   ms_loff_apply<<<blocks, 256, 0, s>>>(B, tot, valid, cmax, csum, bytes);
   return hipGetLastError();
 }
-extern "C" hipError_t scs_launch_ms_fill(const i64* off, const int32_t* adj, const MsUnit* units,
-                                         u32 nunits, const u64* vis, u64 valid, const u32* locid,
+extern "C" hipError_t scs_launch_ms_fill(const int32_t* adj, const MsUnit* units, u32 nunits,
+                                         const u64* vis, u64 valid, const u32* locid,
                                          const MsBatch* B, const u32* hubcnt,
                                          unsigned long long* bytes, hipStream_t s) {
-  ms_fill<<<ms_blocks((u64)nunits * 64, MS_GRID), 256, 0, s>>>(off, adj, units, nunits, vis, valid,
-                                                             locid, B, hubcnt, bytes);
+  ms_fill<<<ms_blocks((u64)nunits * 64, MS_GRID), 256, 0, s>>>(adj, units, nunits, vis, valid,
+                                                              locid, B, hubcnt, bytes);
   return hipGetLastError();
 }
