@@ -49,6 +49,37 @@ def percentile(xs, p):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), p)) if len(xs) else 0.0
 
 
+def prep_leg(scs, G, core, queries):
+    """Batched preprocessing of the queries alone (64 per batch, local graphs
+    built on the device, then released), with no search kernel on the GPU:
+    device time from HIP events and the kernels' algorithmic-byte count."""
+    print("[bench] standalone preprocessing leg", file=sys.stderr, flush=True)
+    EP = scs.Engine(G, limit_queries=64, waves_per_cu=1, mem_budget=32 << 30)
+    core64 = core.astype(np.int64)
+    todo = []
+    for (l, h, q, heu, ld) in queries:
+        kl0, ubd = EP.heuristic(l, h, q)
+        if kl0 < min(int(core64[q]), h - 1):
+            todo.append((q, kl0, ubd))
+    EP.prep_many(*zip(*todo[:1])) if todo else None  # code objects
+    p0 = EP.stats()
+    Ks = []
+    for c0 in range(0, len(todo), 64):
+        qq, kk, uu = zip(*todo[c0:c0 + 64])
+        K, _ = EP.prep_many(qq, kk, uu)
+        Ks.extend(int(x) for x in K)
+    p1 = EP.stats()
+    EP.close()
+    pus = p1["prep_us"] - p0["prep_us"]
+    pby = p1["prep_bytes"] - p0["prep_bytes"]
+    gbs = pby / (pus * 1e-6) / 1e9 if pus > 0 else 0.0
+    return {"queries": len(todo), "batches": int(p1["prep_batches"] - p0["prep_batches"]),
+            "device_us": round(pus, 1), "us_per_query": round(pus / max(1, len(todo)), 2),
+            "algorithmic_bytes": int(pby), "achieved_GBs": round(gbs, 2),
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "mean_kept": round(float(np.mean(Ks)), 1) if Ks else 0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +104,8 @@ def main():
                     help="queries run one at a time (plain order) for the one-query GPU form")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prep-only", action="store_true",
+                    help="only run the standalone batched-preprocessing leg (--prep-sample queries)")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibrate-mib", type=int, default=0,
                     help="only run a plain elementwise pass over this many MiB (read once, "
@@ -110,6 +143,10 @@ def main():
                   limit_queries=args.limit_queries, time_slice_us=args.slice_us,
                   copy_limit=args.copy_limit, mem_budget=32 << 30,
                   heuristic_threads=args.heuristic_threads)
+    if args.prep_only:
+        print(json.dumps({"workload": args.config, "prep_standalone": prep_leg(
+            scs, G, core, queries[:args.prep_sample or len(queries)])}), flush=True)
+        return
     E = scs.Engine(G, **eng_kw)
     if args.learned:
         if model is None:
@@ -238,31 +275,7 @@ def main():
     # batched preprocessing alone (no search kernel sharing the GPU): the
     # roofline of the preprocessing kernels
     if args.prep_sample > 0:
-        print("[bench] standalone preprocessing leg", file=sys.stderr, flush=True)
-        EP = scs.Engine(G, limit_queries=64, waves_per_cu=1, mem_budget=32 << 30)
-        core64 = core.astype(np.int64)
-        todo = []
-        for (l, h, q, heu, ld) in queries[:args.prep_sample]:
-            kl0, ubd = EP.heuristic(l, h, q)
-            if kl0 < min(int(core64[q]), h - 1):
-                todo.append((q, kl0, ubd))
-        EP.prep_many(*zip(*todo[:1])) if todo else None  # code objects
-        p0 = EP.stats()
-        Ks = []
-        for c0 in range(0, len(todo), 64):
-            qq, kk, uu = zip(*todo[c0:c0 + 64])
-            K, _ = EP.prep_many(qq, kk, uu)
-            Ks.extend(int(x) for x in K)
-        p1 = EP.stats()
-        EP.close()
-        pus = p1["prep_us"] - p0["prep_us"]
-        pby = p1["prep_bytes"] - p0["prep_bytes"]
-        gbs = pby / (pus * 1e-6) / 1e9 if pus > 0 else 0.0
-        line["prep_standalone"] = {
-            "queries": len(todo), "batches": int(p1["prep_batches"] - p0["prep_batches"]),
-            "device_us": round(pus, 1), "us_per_query": round(pus / max(1, len(todo)), 2),
-            "algorithmic_bytes": int(pby), "achieved_GBs": round(gbs, 2),
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "mean_kept": round(float(np.mean(Ks)), 1) if Ks else 0}
+        line["prep_standalone"] = prep_leg(scs, G, core, queries[:args.prep_sample])
 
     pb = line["prep"]["algorithmic_bytes"]
     pu = line["prep"]["device_us"]
