@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--limit-queries", type=int, default=2048)
     ap.add_argument("--slice-us", type=int, default=200000)
     ap.add_argument("--copy-limit", type=float, default=0.5)
+    ap.add_argument("--admit-per-step", type=int, default=256,
+                    help="queries admitted (heuristic + GPU preprocessing) per epoch")
     ap.add_argument("--learned", type=int, default=0,
                     help="1 = the main leg itself uses the learned order")
     ap.add_argument("--model", default=os.path.join(ROOT, "models", "branching_c1.npz"),
@@ -142,7 +144,7 @@ def main():
     eng_kw = dict(node_budget=args.budget, waves_per_cu=args.waves_per_cu,
                   limit_queries=args.limit_queries, time_slice_us=args.slice_us,
                   copy_limit=args.copy_limit, mem_budget=32 << 30,
-                  heuristic_threads=args.heuristic_threads)
+                  heuristic_threads=args.heuristic_threads, admit_per_step=args.admit_per_step)
     if args.prep_only:
         print(json.dumps({"workload": args.config, "prep_standalone": prep_leg(
             scs, G, core, queries[:args.prep_sample or len(queries)])}), flush=True)

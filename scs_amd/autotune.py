@@ -11,6 +11,7 @@ queries/s:
                  than copy_limit x waves tasks are pooled (copyLimit)
   limit_tasks    task pool slots (limitTasks)
   time_slice_us  length of one kernel epoch
+  admit_per_step queries taken from the queue per epoch
 Every configuration runs the same queries with the same node budget, so the
 answers it returns are comparable (identical wherever a query completes).
 
@@ -38,6 +39,7 @@ GRID = {
     "time_slice_us": [60000, 200000],
     "limit_tasks": [0],       # task pool slots (the reference's limitTasks); 0 = 2 x waves
     "partition_size": [0],    # per-wave kept-set capacity (partitionSize); 0 = fitted to memory
+    "admit_per_step": [256],  # queries admitted (heuristic + preprocessing) per epoch
 }
 
 
