@@ -57,6 +57,8 @@ hipError_t scs_launch_ms_fill(const int32_t* adj, const MsUnit* units, u32 nunit
                               const u64* vis, u64 valid, const u32* locid, const MsBatch* B,
                               const u32* hubcnt, unsigned long long* bytes, hipStream_t s);
 hipError_t scs_launch_epoch(const EngineArgs* A, hipStream_t s);
+hipError_t scs_launch_slot_write(const QuerySlot* stage, const u32* idx, const u32* wit, u32 n,
+                                 QuerySlot* slots, u32* witness, hipStream_t s);
 hipError_t scs_launch_admit(const EngineArgs* A, const u32* slots, u32 n, u32* failed,
                             hipStream_t s);
 hipError_t scs_launch_pool_init(u64* seq, u32 n, hipStream_t s);
@@ -325,6 +327,16 @@ struct scs_engine {
   u32 trace_cap = 0;
   scs::Arena arena;
   u32* h_tot = nullptr;  // pinned, 192 words
+  // slot hand-over between epochs: admitted slots and their witness seeds
+  // are staged in pinned memory and scattered by one kernel; finished
+  // slots' witnesses come back in one copy
+  QuerySlot* h_stage = nullptr;  // pinned, limit_queries slots
+  u32* h_stage_wit = nullptr;    // pinned, limit_queries x SCS_MAX_H
+  u32* h_stage_idx = nullptr;    // pinned, 2 x limit_queries (slot, witness words)
+  u32* h_wit = nullptr;          // pinned, limit_queries x SCS_MAX_H
+  QuerySlot* d_stage = nullptr;
+  u32* d_stage_wit = nullptr;
+  u32* d_stage_idx = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, pe0 = nullptr, pe1 = nullptr;
   // host
   std::mutex qmu;
@@ -499,6 +511,17 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   if (hipMalloc(&E->d_prepbytes, prepbytes_len) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipMalloc(&E->d_arena, p.mem_budget) != hipSuccess) return fail(SCS_ENOMEM);
   if (hipHostMalloc(&E->h_tot, sizeof(u32) * 192) != hipSuccess) return fail(SCS_ENOMEM);
+  {
+    const u64 wit_words = (u64)nq * SCS_MAX_H;
+    if (hipHostMalloc(&E->h_stage, sizeof(QuerySlot) * nq) != hipSuccess ||
+        hipHostMalloc(&E->h_stage_wit, sizeof(u32) * wit_words) != hipSuccess ||
+        hipHostMalloc(&E->h_stage_idx, sizeof(u32) * 2 * nq) != hipSuccess ||
+        hipHostMalloc(&E->h_wit, sizeof(u32) * wit_words) != hipSuccess ||
+        hipMalloc(&E->d_stage, sizeof(QuerySlot) * nq) != hipSuccess ||
+        hipMalloc(&E->d_stage_wit, sizeof(u32) * wit_words) != hipSuccess ||
+        hipMalloc(&E->d_stage_idx, sizeof(u32) * 2 * nq) != hipSuccess)
+      return fail(SCS_ENOMEM);
+  }
   E->arena.init(E->d_arena, p.mem_budget);
   if (hipMemset(E->d_slots, 0, sizeof(QuerySlot) * nq) != hipSuccess) return fail(SCS_EHIP);
   if (hipMemset(E->d_wave, 0, lay.stride * E->nwaves) != hipSuccess) return fail(SCS_EHIP);
@@ -605,6 +628,10 @@ extern "C" void scs_engine_destroy(scs_engine* E) {
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (E->h_tot) (void)hipHostFree(E->h_tot);
+  for (void* q : {(void*)E->h_stage, (void*)E->h_stage_wit, (void*)E->h_stage_idx, (void*)E->h_wit})
+    if (q) (void)hipHostFree(q);
+  for (void* q : {(void*)E->d_stage, (void*)E->d_stage_wit, (void*)E->d_stage_idx})
+    if (q) (void)hipFree(q);
   if (E->h_batch) (void)hipHostFree(E->h_batch);
   for (hipEvent_t ev : {E->ev0, E->ev1, E->pe0, E->pe1})
     if (ev) (void)hipEventDestroy(ev);
@@ -853,19 +880,27 @@ static void release_slot(scs_engine* E, u32 si) {
   hs.used = false;
 }
 
+// Results of the queries whose slots the last epoch marked done.  Runs
+// between epochs (no kernel touches the slots): one copy brings the slots,
+// one the witness rows of the finished ones, and one writes the slot array
+// back with those slots marked free.
 static int collect(scs_engine* E) {
   const u32 nq = E->p.limit_queries;
   std::vector<QuerySlot> dev(nq);
   HIPCHK(hipMemcpy(dev.data(), E->d_slots, sizeof(QuerySlot) * nq, hipMemcpyDeviceToHost));
+  std::vector<u32> fin;
+  for (u32 si = 0; si < nq; si++)
+    if (E->slots[si].used && dev[si].state == QS_DONE) fin.push_back(si);
+  if (fin.empty()) return SCS_OK;
+  const u64 r0 = fin.front(), r1 = (u64)fin.back() + 1;  // witness rows to bring back
+  HIPCHK(hipMemcpy(E->h_wit + r0 * SCS_MAX_H, E->d_witness + r0 * SCS_MAX_H,
+                   sizeof(u32) * SCS_MAX_H * (r1 - r0), hipMemcpyDeviceToHost));
   std::vector<u32> wit(SCS_MAX_H);
-  for (u32 si = 0; si < nq; si++) {
+  for (u32 si : fin) {
     HostSlot& hs = E->slots[si];
-    if (!hs.used || dev[si].state != QS_DONE) continue;
-    const QuerySlot& Q = dev[si];
+    QuerySlot& Q = dev[si];
     u32 ws = std::min<u32>(Q.wsize, SCS_MAX_H);
-    if (ws)
-      HIPCHK(hipMemcpy(wit.data(), E->d_witness + (u64)si * SCS_MAX_H, sizeof(u32) * ws,
-                       hipMemcpyDeviceToHost));
+    if (ws) memcpy(wit.data(), E->h_wit + (u64)si * SCS_MAX_H, sizeof(u32) * ws);
     int status;
     if (Q.capped)
       status = SCS_ST_CAPPED;
@@ -881,11 +916,10 @@ static int collect(scs_engine* E) {
     push_result(E, hs.pq, Q.kl, hs.kl0, hs.ku, status, hs.K, Q.nodes, wit.data(), ws);
     E->st.nodes += Q.nodes;
     release_slot(E, si);
-    // mark free on the device
-    u32 fr = QS_FREE;
-    HIPCHK(hipMemcpy(&E->d_slots[si].state, &fr, sizeof(u32), hipMemcpyHostToDevice));
+    Q.state = QS_FREE;
     E->active--;
   }
+  HIPCHK(hipMemcpy(E->d_slots, dev.data(), sizeof(QuerySlot) * nq, hipMemcpyHostToDevice));
   return SCS_OK;
 }
 
@@ -1041,14 +1075,24 @@ extern "C" int scs_step(scs_engine* E) {
   }
   // 5. admitted queries become visible to the search (engine stream)
   if (!admitted.empty()) {
-    for (u32 si : admitted) {
+    const u32 na = (u32)admitted.size();
+    for (u32 i = 0; i < na; i++) {
+      const u32 si = admitted[i];
       const HostSlot& hs = E->slots[si];
-      const QuerySlot& Q = E->h_slots[si];
-      if (!hs.seed_witness.empty())
-        HIPCHK(hipMemcpyAsync(Q.witness, hs.seed_witness.data(), sizeof(u32) * hs.seed_witness.size(),
-                              hipMemcpyHostToDevice, E->s));
-      HIPCHK(hipMemcpyAsync(&E->d_slots[si], &Q, sizeof(QuerySlot), hipMemcpyHostToDevice, E->s));
+      const u32 ws = (u32)std::min<size_t>(hs.seed_witness.size(), SCS_MAX_H);
+      E->h_stage[i] = E->h_slots[si];
+      E->h_stage_idx[2 * i] = si;
+      E->h_stage_idx[2 * i + 1] = ws;
+      if (ws) memcpy(E->h_stage_wit + (u64)i * SCS_MAX_H, hs.seed_witness.data(), sizeof(u32) * ws);
     }
+    HIPCHK(hipMemcpyAsync(E->d_stage, E->h_stage, sizeof(QuerySlot) * na, hipMemcpyHostToDevice,
+                          E->s));
+    HIPCHK(hipMemcpyAsync(E->d_stage_idx, E->h_stage_idx, sizeof(u32) * 2 * na,
+                          hipMemcpyHostToDevice, E->s));
+    HIPCHK(hipMemcpyAsync(E->d_stage_wit, E->h_stage_wit, sizeof(u32) * SCS_MAX_H * (u64)na,
+                          hipMemcpyHostToDevice, E->s));
+    HIPCHK(scs_launch_slot_write(E->d_stage, E->d_stage_idx, E->d_stage_wit, na, E->d_slots,
+                                 E->d_witness, E->s));
     HIPCHK(hipMemcpyAsync(E->d_admit, admitted.data(), sizeof(u32) * admitted.size(),
                           hipMemcpyHostToDevice, E->s));
     HIPCHK(scs_launch_admit(&E->args, E->d_admit, (u32)admitted.size(),

@@ -1785,6 +1785,30 @@ __global__ void __launch_bounds__(64) scs_admit(EngineArgs A, const u32* slots_i
   }
 }
 
+// Admitted slots, staged by the host: block i writes slot idx[2i] from
+// stage[i] and its idx[2i+1] witness seed words.  Runs between epochs.
+__global__ void __launch_bounds__(64) scs_slot_write(const QuerySlot* __restrict__ stage,
+                                                     const u32* __restrict__ idx,
+                                                     const u32* __restrict__ wit, u32 n,
+                                                     QuerySlot* __restrict__ slots,
+                                                     u32* __restrict__ witness) {
+  const u32 i = blockIdx.x;
+  if (i >= n) return;
+  const u32 si = idx[2 * i], ws = idx[2 * i + 1];
+  const u32* src = (const u32*)&stage[i];
+  u32* dst = (u32*)&slots[si];
+  for (u32 k = LANE; k < sizeof(QuerySlot) / 4; k += 64) dst[k] = src[k];
+  for (u32 k = LANE; k < ws && k < SCS_MAX_H; k += 64)
+    witness[(u64)si * SCS_MAX_H + k] = wit[(u64)i * SCS_MAX_H + k];
+}
+extern "C" hipError_t scs_launch_slot_write(const QuerySlot* stage, const u32* idx, const u32* wit,
+                                            u32 n, QuerySlot* slots, u32* witness,
+                                            hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  scs_slot_write<<<n, 64, 0, s>>>(stage, idx, wit, n, slots, witness);
+  return hipGetLastError();
+}
+
 extern "C" hipError_t scs_launch_admit(const EngineArgs* A, const u32* slots, u32 n, u32* failed,
                                        hipStream_t s) {
   scs_admit<<<1, 64, 0, s>>>(*A, slots, n, failed);
