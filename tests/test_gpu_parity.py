@@ -26,7 +26,32 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 GOLDEN = json.load(open(os.path.join(GOLD, "golden.json")))
 
 
+def hub_graph():
+    """3,000 vertices: a sparse random graph plus three hubs of degree about
+    2,600, 1,300 and 700 (ids 0-2), so adjacency lists span many 256-entry
+    units and exceed the whole-wave BFS threshold."""
+    rng = np.random.default_rng(7)
+    n = 3000
+    e = rng.integers(0, n, size=(12000, 2))
+    hubs = [np.stack([np.full(k, h), rng.choice(np.arange(3, n), k, replace=False)], 1)
+            for h, k in ((0, 2600), (1, 1300), (2, 700))]
+    e = np.concatenate([e] + hubs + [np.array([[0, 1], [1, 2], [0, 2]])])
+    e = e[e[:, 0] != e[:, 1]]
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    src = np.concatenate([e[:, 0], e[:, 1]])
+    dst = np.concatenate([e[:, 1], e[:, 0]])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off)
+    return n, off, dst.astype(np.int32)
+
+
 def load_pair(name):
+    if name == "hubs":
+        n, off, adj = hub_graph()
+        return scs.Graph.from_csr(off, adj), OracleGraph.from_csr(n, off, adj)
     if name == "C1":
         g = configs.graph("C1")
         return scs.Graph.from_csr(g.off, g.adj), OracleGraph.from_csr(g.n, g.off, g.adj)
@@ -36,7 +61,7 @@ def load_pair(name):
 
 @pytest.fixture(scope="module")
 def graphs():
-    out = {n: load_pair(n) for n in ("graph.txt", "100.txt", "hepPH_SCS", "C1")}
+    out = {n: load_pair(n) for n in ("graph.txt", "100.txt", "hepPH_SCS", "C1", "hubs")}
     yield out
     for G, _ in out.values():
         G.close()
@@ -88,7 +113,7 @@ def test_preprocessing_matches_oracle(graphs, name):
     E.close()
 
 
-@pytest.mark.parametrize("name", ["hepPH_SCS", "C1"])
+@pytest.mark.parametrize("name", ["hepPH_SCS", "C1", "hubs"])
 def test_batched_preprocessing_matches_oracle(graphs, name):
     """64 queries with mixed kl0 (including kl0 <= 1, whose depth bound is
     h - 1) in one batched pass: every query's local graph equals the
@@ -101,6 +126,7 @@ def test_batched_preprocessing_matches_oracle(graphs, name):
     n = og.n
     cnt = 64
     qs = rng.integers(0, n, cnt).astype(np.uint32)
+    qs[:3] = [0, 1, 2]  # seeds at the hubs of the hub graph (ordinary vertices elsewhere)
     kl0 = np.array([int(rng.integers(0, max(1, int(core[q])))) if i % 4 else 0 for i, q in enumerate(qs)],
                    np.uint32)
     hs = rng.integers(3, 40, cnt)
