@@ -622,7 +622,9 @@ __global__ void __launch_bounds__(256) ms_loff_apply(const MsBatch* __restrict__
 // a load instruction reads JT pieces of QT entries of the locid rows, where
 // a whole-row or whole-column form touches 64 lines with one of the two.
 // On C2 (Chung-Lu, 100K vertices) 8 x 8 tiles with 32 row loads in flight
-// were fastest; 4 x 16 and 2 x 32 were 1 % and 22 % slower.
+// were fastest; 4 x 16 and 2 x 32 were 1 % and 22 % slower, non-temporal
+// stores (no write-combining in the L2) 2.7x slower, and runs of 2 to 8
+// consecutive units per wave 3-6 % slower.
 #ifndef MS_FILL_QT
 #define MS_FILL_QT 8
 #endif
