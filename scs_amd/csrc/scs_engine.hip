@@ -1023,8 +1023,10 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   wsync();
   const bool single0 = (Cs == 1 && w.vs[w.clist[0]].dC == 0);
   if (w.bm) {
-    // tb = C, so "x is R" is two LDS bit tests
-    for (u32 i = LANE; i < Cs; i += 64) tset(w, w.clist[i]);
+    // C is marked in xb for the scatter, so "x is R" is one LDS bit test
+    // (C and X neighbours are skipped alike); tb records which R vertices
+    // the scatter has reached
+    for (u32 i = LANE; i < Cs; i += 64) atomicOr(&w.xb[w.clist[i] >> 5], 1u << (w.clist[i] & 31));
     wsync();
   }
   // The adjacency lists of C, concatenated: 64 C vertices at a time, each
@@ -1051,10 +1053,9 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     }
     const u32 tot = __shfl(inc, 63);
     const u32 exc = inc - ud;
-    for (u32 k0 = 0; k0 < tot; k0 += 64 * U) {
-      u32 x[U];
-      u64 rr[U];
-      bool ok[U];
+    // entry kk of the concatenated lists (clamped to the last one): its
+    // neighbour id and its owner's score term
+    auto fetch = [&](u32 k0, u32* x, u64* rr) {
 #pragma unroll
       for (int k = 0; k < U; k++) {
         const u32 kk = k0 + 64 * k + LANE;
@@ -1068,26 +1069,64 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
         const u32 ob = __shfl(ub, (int)lo), oe = __shfl(exc, (int)lo);
         const u32 rlo = __shfl((u32)ur, (int)lo), rhi = __shfl((u32)(ur >> 32), (int)lo);
         rr[k] = ((u64)rhi << 32) | rlo;
-        x[k] = kk < tot ? w.adj[ob + (q - oe)] : 0u;
+        x[k] = w.adj[ob + (q - oe)];
       }
+    };
+    if (w.bm && tot > 0) {
+      // Software-pipelined: the next step's adjacency loads are issued
+      // before this step's score atomics, which return nothing, so no step
+      // waits for an atomic (the memory counter is in order: a load issued
+      // after an atomic would wait for it).  First touch of an R vertex is
+      // an LDS test-and-set on tb.
+      u32 xa[U], xn[U];
+      u64 ra[U], rn[U];
+      fetch(0, xa, ra);
+      for (u32 k0 = 0; k0 < tot; k0 += 64 * U) {
+        fetch(k0 + 64 * U < tot ? k0 + 64 * U : k0, xn, rn);
+        bool ok[U];
 #pragma unroll
-      for (int k = 0; k < U; k++)
-        ok[k] = k0 + 64 * k + LANE < tot &&
-                (w.bm ? !(xbit(w, x[k]) || tbit(w, x[k])) : w.vs[x[k]].st == ST_R);
-      u64 old[U];
+        for (int k = 0; k < U; k++) ok[k] = k0 + 64 * k + LANE < tot && !xbit(w, xa[k]);
 #pragma unroll
-      for (int k = 0; k < U; k++)
-        old[k] = ok[k] ? atomicAdd((unsigned long long*)&w.vs[x[k]].acc, (unsigned long long)rr[k])
-                       : 1ull;
+        for (int k = 0; k < U; k++)
+          if (ok[k]) atomicAdd((unsigned long long*)&w.vs[xa[k]].acc, (unsigned long long)ra[k]);
 #pragma unroll
-      for (int k = 0; k < U; k++)
-        if (old[k] == 0) w.front[atomicAdd(&L->nfront, 1u)] = x[k];
+        for (int k = 0; k < U; k++)
+          if (ok[k]) {
+            const u32 bit = 1u << (xa[k] & 31);
+            if (!(atomicOr(&w.tb[xa[k] >> 5], bit) & bit)) w.front[atomicAdd(&L->nfront, 1u)] = xa[k];
+          }
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+          xa[k] = xn[k];
+          ra[k] = rn[k];
+        }
+      }
+    } else if (tot > 0) {
+      for (u32 k0 = 0; k0 < tot; k0 += 64 * U) {
+        u32 x[U];
+        u64 rr[U];
+        fetch(k0, x, rr);
+        bool ok[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) ok[k] = k0 + 64 * k + LANE < tot && w.vs[x[k]].st == ST_R;
+        u64 old[U];
+#pragma unroll
+        for (int k = 0; k < U; k++)
+          old[k] = ok[k] ? atomicAdd((unsigned long long*)&w.vs[x[k]].acc, (unsigned long long)rr[k])
+                         : 1ull;
+#pragma unroll
+        for (int k = 0; k < U; k++)
+          if (old[k] == 0) w.front[atomicAdd(&L->nfront, 1u)] = x[k];
+      }
     }
     w.edges += tot;
   }
+  // the barrier's fence waits for the score atomics
   wsync();
   if (w.bm) {
-    for (u32 i = LANE; i < Cs; i += 64) tclr(w, w.clist[i]);
+    for (u32 i = LANE; i < Cs; i += 64) atomicAnd(&w.xb[w.clist[i] >> 5], ~(1u << (w.clist[i] & 31)));
+    const u32 nfr = L->nfront;
+    for (u32 i = LANE; i < nfr; i += 64) w.tb[w.front[i] >> 5] = 0u;  // tb was all zero before
     wsync();
   }
   const u32 nf = bcast(L->nfront);
