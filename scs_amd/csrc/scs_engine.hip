@@ -444,8 +444,23 @@ __device__ __forceinline__ void dec_neighbour(W& w, u32 x, const Clo& c) {
 }
 
 // Decrements dR of every live neighbour of list[0..m) (load-balanced like
-// scan_adj); on the per-lane path four decrements are in flight before
-// their returned words are checked.
+// scan_adj).  Four entries per lane per step, software-pipelined: the next
+// step's adjacency loads are issued before this step's decrements, so one
+// memory round trip covers both (the memory counter is in order).  Loads
+// past a list's end read its last entry (or entry 0 of an empty graph) and
+// are ignored.
+__device__ __forceinline__ void dec4(W& w, const u32 x[4], const bool in[4], const Clo& c) {
+  bool lv[4];
+  u64 od[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) lv[k] = in[k] && dec_live(w, x[k]);
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    od[k] = lv[k] ? atomicSub((unsigned long long*)&w.vs[x[k]], 1ull) : 0ull;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (lv[k]) dec_check(w, x[k], od[k], c);
+}
 __device__ __forceinline__ void dec_neighbours(W& w, const u32* list, u32 m, const Clo& c) {
   u32 ed = 0;
   for (u32 c0 = 0; c0 < m; c0 += 64) {
@@ -457,24 +472,21 @@ __device__ __forceinline__ void dec_neighbours(W& w, const u32* list, u32 m, con
       e = w.off[v + 1];
     }
     const bool big = i < m && (e - b) > SEQ_DEG;
-    if (i < m && !big) {
-      u32 j = b;
-      for (; j + 4 <= e; j += 4) {
-        u32 x[4];
-        bool lv[4];
-        u64 od[4];
+    if (i < m && !big && e > b) {
+      u32 x[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) x[k] = w.adj[j + k];
+      for (int k = 0; k < 4; k++) x[k] = w.adj[min(b + k, e - 1)];
+      for (u32 j = b; j < e; j += 4) {
+        u32 xn[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) lv[k] = dec_live(w, x[k]);
+        for (int k = 0; k < 4; k++) xn[k] = w.adj[min(j + 4 + k, e - 1)];
+        bool in[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-          od[k] = lv[k] ? atomicSub((unsigned long long*)&w.vs[x[k]], 1ull) : 0ull;
+        for (int k = 0; k < 4; k++) in[k] = j + k < e;
+        dec4(w, x, in, c);
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-          if (lv[k]) dec_check(w, x[k], od[k], c);
+        for (int k = 0; k < 4; k++) x[k] = xn[k];
       }
-      for (; j < e; j++) dec_neighbour(w, w.adj[j], c);
       ed += e - b;
     }
     u64 bm = __ballot(big);
@@ -482,7 +494,21 @@ __device__ __forceinline__ void dec_neighbours(W& w, const u32* list, u32 m, con
       const int src = __ffsll((unsigned long long)bm) - 1;
       bm &= bm - 1;
       const u32 bb = __shfl(b, src), be = __shfl(e, src);
-      for (u32 j = bb + LANE; j < be; j += 64) dec_neighbour(w, w.adj[j], c);
+      // the whole wave, 256 entries per step
+      u32 x[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) x[k] = w.adj[min(bb + LANE + 64 * k, be - 1)];
+      for (u32 j = bb; j < be; j += 256) {
+        u32 xn[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) xn[k] = w.adj[min(j + 256 + LANE + 64 * k, be - 1)];
+        bool in[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) in[k] = j + LANE + 64 * k < be;
+        dec4(w, x, in, c);
+#pragma unroll
+        for (int k = 0; k < 4; k++) x[k] = xn[k];
+      }
       if (LANE == 0) ed += be - bb;
     }
   }
