@@ -326,35 +326,52 @@ __device__ __forceinline__ void hist_add(u32* hist, bool on, u32 d, int delta) {
 // marks the batch X; its undo marks the batch R first and then increments,
 // so batch members see each other both ways.
 // ---------------------------------------------------------------------------
-// U: adjacency chunks in flight per step (the learned kernel variant uses 1:
-// it has no registers to spare).
-template <int U = 4>
-__device__ void move_c(W& w, u32 v) {
+// Adds delta (+1 or -1) to dC and -delta to dR of every live neighbour of
+// v, keeping the dC histogram of R.  U chunks of 64 entries per step,
+// software-pipelined: a step issues its state gathers and the next step's
+// adjacency loads together, then writes {dR, dC, status, queued} as one
+// 64-bit word (v's neighbours are distinct, so no lane shares a record).
+template <int U>
+__device__ __forceinline__ void shift_dc(W& w, u32 v, int delta) {
   const u32 b = w.off[v], e = w.off[v + 1];
   u32* hist = w.L->hist;
-  for (u32 j = b + LANE; j < e; j += 64 * U) {
+  if (e > b) {
     u32 x[U];
-    VState vs[U];
 #pragma unroll
-    for (int k = 0; k < U; k++) x[k] = j + 64 * k < e ? w.adj[j + 64 * k] : INV;
+    for (int k = 0; k < U; k++) x[k] = w.adj[min(b + LANE + 64 * k, e - 1)];
+    for (u32 j = b; j < e; j += 64 * U) {
+      u64 wd[U];  // {dR, dC, status, queued} of each neighbour
+      u32 xn[U];
 #pragma unroll
-    for (int k = 0; k < U; k++)
-      if (x[k] != INV) vs[k] = w.vs[x[k]];
+      for (int k = 0; k < U; k++) wd[k] = *(const u64*)&w.vs[x[k]];
 #pragma unroll
-    for (int k = 0; k < U; k++) {
-      const bool live = x[k] != INV && vs[k].st != ST_X;
-      const u32 dc = live ? vs[k].dC : 0u;
-      const bool r = live && vs[k].st == ST_R;
-      hist_add(hist, r, dc, -1);
-      hist_add(hist, r, dc + 1, 1);
-      if (live) {
-        w.vs[x[k]].dC = (u16)(dc + 1);
-        w.vs[x[k]].dR = vs[k].dR - 1;
+      for (int k = 0; k < U; k++) xn[k] = w.adj[min(j + 64 * U + LANE + 64 * k, e - 1)];
+#pragma unroll
+      for (int k = 0; k < U; k++) {
+        const u32 st = (u32)(wd[k] >> 48) & 0xffu;
+        const bool live = j + LANE + 64 * k < e && st != ST_X;
+        const u32 dc = live ? (u32)(wd[k] >> 32) & 0xffffu : (delta > 0 ? 0u : 1u);
+        const bool r = live && st == ST_R;
+        hist_add(hist, r, dc, -1);
+        hist_add(hist, r, dc + delta, 1);
+        if (live)
+          *(u64*)&w.vs[x[k]] = (wd[k] & 0xFFFF000000000000ull) | ((u64)(u16)(dc + delta) << 32) |
+                               (u32)((u32)wd[k] - delta);
       }
+#pragma unroll
+      for (int k = 0; k < U; k++) x[k] = xn[k];
     }
   }
   w.edges += e - b;
   wsync();
+}
+
+// U: adjacency chunks in flight per step (the learned kernel variant uses 1:
+// it has no registers to spare).
+template <int U = 4>
+__device__ void move_c(W& w, u32 v) {
+  shift_dc<U>(w, v, 1);
+  u32* hist = w.L->hist;
   if (LANE == 0) {
     w.vs[v].st = ST_C;
     hist[w.vs[v].dC] -= 1;
@@ -383,30 +400,7 @@ __device__ void undo_move(W& w, u32 v) {
   w.nR++;
   w.nC--;
   wsync();
-  const u32 b = w.off[v], en = w.off[v + 1];
-  for (u32 j = b + LANE; j < en; j += 64 * U) {
-    u32 x[U];
-    VState vs[U];
-#pragma unroll
-    for (int k = 0; k < U; k++) x[k] = j + 64 * k < en ? w.adj[j + 64 * k] : INV;
-#pragma unroll
-    for (int k = 0; k < U; k++)
-      if (x[k] != INV) vs[k] = w.vs[x[k]];
-#pragma unroll
-    for (int k = 0; k < U; k++) {
-      const bool live = x[k] != INV && vs[k].st != ST_X;
-      const u32 dc = live ? vs[k].dC : 1u;
-      const bool r = live && vs[k].st == ST_R;
-      hist_add(hist, r, dc, -1);
-      hist_add(hist, r, dc - 1, 1);
-      if (live) {
-        w.vs[x[k]].dC = (u16)(dc - 1);
-        w.vs[x[k]].dR = vs[k].dR + 1;
-      }
-    }
-  }
-  w.edges += en - b;
-  wsync();
+  shift_dc<U>(w, v, -1);
 }
 
 // Closure parameters for crossing detection during a batch (rule 1).
