@@ -81,8 +81,9 @@ typedef struct scs_params {
   uint32_t red1, red2, red3;  /* argv[8..10]: reduction rules 1-3 */
   uint32_t prun1, prun2;      /* argv[11..12]: bounds U_d and U_nr */
   /* named extensions */
-  uint32_t waves_per_cu;   /* resident search waves per CU (0 = 4, max 16); at 4 each
-                              wave has room for LDS bitmaps over 130K kept vertices */
+  uint32_t waves_per_cu;   /* resident search waves per CU (0 = 4, max 8: the search
+                              kernel's registers allow 8 resident waves per CU); at 4
+                              each wave has room for LDS bitmaps over 130K kept vertices */
   uint32_t limit_tasks;    /* task pool slots (0 = 4 * waves) */
   uint64_t node_budget;    /* per-query node cap before CAPPED (0 = none) */
   uint64_t mem_budget;     /* bytes of device memory for per-query graphs */

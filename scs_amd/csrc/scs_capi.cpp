@@ -406,7 +406,10 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
     scs_params_default(&p);
   if (p.limit_queries == 0) p.limit_queries = 1;
   if (p.waves_per_cu == 0) p.waves_per_cu = 4;
-  if (p.waves_per_cu > 16) p.waves_per_cu = 16;  // LDS per wave (~8 KB) x 16 fits 160 KB
+  // the search kernel is built for at least 2 waves per SIMD (up to 256
+  // registers): 8 per CU are resident, and every wave of the persistent
+  // grid must be, or the states of the others would never run
+  if (p.waves_per_cu > 8) p.waves_per_cu = 8;
   if (p.time_slice_us == 0) p.time_slice_us = 200000;
   if (p.heuristic_threads == 0) p.heuristic_threads = 8;
   if (p.admit_per_step == 0) p.admit_per_step = 256;

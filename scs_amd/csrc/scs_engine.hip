@@ -366,8 +366,7 @@ __device__ __forceinline__ void shift_dc(W& w, u32 v, int delta) {
   wsync();
 }
 
-// U: adjacency chunks in flight per step (the learned kernel variant uses 1:
-// it has no registers to spare).
+// U: adjacency chunks in flight per step.
 template <int U = 4>
 __device__ void move_c(W& w, u32 v) {
   shift_dc<U>(w, v, 1);
@@ -1000,7 +999,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
         for (u32 j = b; j < e; j++) {
           u32 x = w.adj[j];
           if (w.vs[x].st == ST_R) {
-            move_c<LEARN ? 1 : 4>(w, x);
+            move_c<4>(w, x);
             if (w.nC > h) {
               over = true;
               break;
@@ -1054,8 +1053,8 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   // walks the concatenated lists 64 entries per chunk; the owner of an
   // entry is found by binary search over the lanes' exclusive degree sums.
   // U chunks per step keep their loads, tests and atomics in flight
-  // together (the learned variant has no registers to spare for U > 1).
-  constexpr int U = LEARN ? 1 : 4;
+  // together.
+  constexpr int U = 4;
   for (u32 c0 = 0; c0 < Cs; c0 += 64) {
     const u32 ci = c0 + LANE;
     u32 ub = 0, ud = 0;
@@ -1162,9 +1161,8 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   };
   {
     u32 i = LANE;
-    // four candidates per lane per step: their gathers overlap (the
-    // learned variant has no registers to spare for it)
-    for (; !LEARN && i + 192 < nf; i += 256) {
+    // four candidates per lane per step: their gathers overlap
+    for (; i + 192 < nf; i += 256) {
       const u32 x0 = w.front[i], x1 = w.front[i + 64], x2 = w.front[i + 128],
                 x3 = w.front[i + 192];
       const VState v0 = w.vs[x0], v1 = w.vs[x1], v2 = w.vs[x2], v3 = w.vs[x3];
@@ -1560,9 +1558,12 @@ __device__ void flush_counters(W& w, const EngineArgs& A, u32& pend) {
 // The persistent epoch kernel.
 // ---------------------------------------------------------------------------
 // flatten: every call is inlined, so the wave state W never needs an address
-// in private (scratch) memory
+// in private (scratch) memory.  Launch bounds: at least 2 waves per SIMD,
+// i.e. up to 256 registers: the pipelined walks keep two steps of loads and
+// atomics in registers (at 128 they spilled), and the engine runs at most 8
+// waves per CU.
 template <bool LEARN>
-__global__ void __launch_bounds__(64, 4) __attribute__((flatten)) scs_epoch(EngineArgs A) {
+__global__ void __launch_bounds__(64, 2) __attribute__((flatten)) scs_epoch(EngineArgs A) {
   __shared__ Lds L;
   const u32 wave = blockIdx.x;
   if (wave >= A.nwaves) return;
@@ -1690,13 +1691,13 @@ __global__ void __launch_bounds__(64, 4) __attribute__((flatten)) scs_epoch(Engi
       kl_seen = o.klc;
       const Clo none{0, 0, nullptr, nullptr};
       remove_batch(w, f->D, o.nD, none);
-      move_c<LEARN ? 1 : 4>(w, o.ustar);
+      move_c<4>(w, o.ustar);
     } else {
       // backtrack to the next pending child
       bool cont = false;
       while (w.nframes > 0) {
         Frame* f = &w.frames[w.nframes - 1];
-        undo_to<LEARN ? 1 : 4>(w, f->mark);
+        undo_to<4>(w, f->mark);
         from = f->mark;
         kl_seen = f->kl;
         const u32 next = f->next + 1;
@@ -1707,8 +1708,8 @@ __global__ void __launch_bounds__(64, 4) __attribute__((flatten)) scs_epoch(Engi
         if (next <= nD) {
           const u32 i = next - 1;
           remove_batch(w, f->D, i, none);
-          move_c<LEARN ? 1 : 4>(w, f->ustar);
-          move_c<LEARN ? 1 : 4>(w, f->D[i]);
+          move_c<4>(w, f->ustar);
+          move_c<4>(w, f->D[i]);
           cont = true;
           break;
         }
