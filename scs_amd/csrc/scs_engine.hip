@@ -1054,7 +1054,10 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   // entry is found by binary search over the lanes' exclusive degree sums.
   // U chunks per step keep their loads, tests and atomics in flight
   // together.
-  constexpr int U = 4;
+#ifndef SCS_SCORE_U
+#define SCS_SCORE_U 4
+#endif
+  constexpr int U = SCS_SCORE_U;
   for (u32 c0 = 0; c0 < Cs; c0 += 64) {
     const u32 ci = c0 + LANE;
     u32 ub = 0, ud = 0;
