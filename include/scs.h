@@ -97,7 +97,10 @@ typedef struct scs_params {
   uint32_t heuristic_threads; /* host threads for CSSC_heu (0 = 8) */
   uint32_t admit_per_step; /* queries admitted per scs_step (0 = 256): the
                               search starts while later arrivals are prepared */
-  uint32_t reserved[6];
+  uint32_t no_lds_bitmaps; /* 1 = keep vertex status in device memory only, even for
+                              kept sets whose bitmaps fit in LDS (the path large kept
+                              sets take; for testing it on small graphs) */
+  uint32_t reserved[5];
 } scs_params;
 
 typedef struct scs_result {

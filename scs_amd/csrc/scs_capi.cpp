@@ -554,6 +554,7 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   A.deterministic = p.deterministic;
   A.node_flush = p.deterministic ? 1 : 16;
   A.waves_per_cu = p.deterministic ? 1 : p.waves_per_cu;
+  A.no_bitmaps = p.no_lds_bitmaps ? 1u : 0u;
   E->slots.resize(nq);
   E->h_slots.resize(nq);
   E->heu_work.resize(p.heuristic_threads);

@@ -1812,7 +1812,7 @@ static u32 epoch_lds_bytes(u32 waves_per_cu) {
 extern "C" hipError_t scs_launch_epoch(const EngineArgs* A, hipStream_t s) {
   EngineArgs a = *A;
   const u32 dyn = epoch_lds_bytes(a.waves_per_cu);
-  a.lds_words = dyn / 4;
+  a.lds_words = a.no_bitmaps ? 0u : dyn / 4;
   if (a.model)
     scs_epoch<true><<<a.nwaves, 64, dyn, s>>>(a);
   else

@@ -204,5 +204,5 @@ struct EngineArgs {
   u32 node_flush;      // flush node counters every this many nodes
   u32 waves_per_cu;    // resident waves per CU the launch is sized for
   u32 lds_words;       // dynamic LDS words per wave (set by the launcher)
-  u32 pad;
+  u32 no_bitmaps;      // 1 = no LDS bitmaps (status from device memory only)
 };

@@ -45,7 +45,8 @@ class Params(C.Structure):
                 ("max_kept", C.c_uint32), ("time_slice_us", C.c_uint32),
                 ("deterministic", C.c_uint32), ("learned", C.c_uint32),
                 ("model", C.c_void_p), ("heuristic_threads", C.c_uint32),
-                ("admit_per_step", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+                ("admit_per_step", C.c_uint32), ("no_lds_bitmaps", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
 
 
 class Result(C.Structure):

@@ -164,8 +164,11 @@ def run_engine(G, queries, **kw):
     return res, st
 
 
+@pytest.mark.parametrize("no_bitmaps", [0, 1])
 @pytest.mark.parametrize("name", ["graph.txt", "100.txt", "hepPH_SCS", "C1"])
-def test_deterministic_mode_expands_the_oracles_nodes(graphs, name):
+def test_deterministic_mode_expands_the_oracles_nodes(graphs, name, no_bitmaps):
+    """no_bitmaps=1 runs the device-memory status path that kept sets too
+    large for the LDS bitmaps take."""
     G, og = graphs[name]
     if name == "C1":
         core = og.core()
@@ -175,7 +178,8 @@ def test_deterministic_mode_expands_the_oracles_nodes(graphs, name):
     else:
         qs = [tuple(r["query"]) for r in GOLDEN[name] if r["nodes"] <= 20000]
         cap = 20000
-    res, _ = run_engine(G, qs, deterministic=1, node_budget=cap, limit_queries=16)
+    res, _ = run_engine(G, qs, deterministic=1, node_budget=cap, limit_queries=16,
+                        no_lds_bitmaps=no_bitmaps)
     for r, q in zip(res, qs):
         o = og.solve(*q, node_cap=cap)
         assert (r.kl, r.status, r.nodes) == (o["kl"], o["status"], o["nodes"]), (name, q)
