@@ -1124,22 +1124,44 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
         }
       }
     } else if (tot > 0) {
+      // Status from device memory, first touch from the returned old score.
+      // Pipelined over two steps: a step issues the next step's adjacency
+      // loads with its status gathers, then its score atomics, and only then
+      // appends the previous step's first touches, whose returned words
+      // have arrived by the time this step's gathers have.
+      u32 xa[U], xp[U];
+      u64 ra[U], op[U];
+      bool okp[U];
+      fetch(0, xa, ra);
+#pragma unroll
+      for (int k = 0; k < U; k++) okp[k] = false;
       for (u32 k0 = 0; k0 < tot; k0 += 64 * U) {
-        u32 x[U];
-        u64 rr[U];
-        fetch(k0, x, rr);
+        u32 xn[U];
+        u64 rn[U];
+        fetch(k0 + 64 * U < tot ? k0 + 64 * U : k0, xn, rn);
         bool ok[U];
 #pragma unroll
-        for (int k = 0; k < U; k++) ok[k] = k0 + 64 * k + LANE < tot && w.vs[x[k]].st == ST_R;
+        for (int k = 0; k < U; k++) ok[k] = k0 + 64 * k + LANE < tot && w.vs[xa[k]].st == ST_R;
         u64 old[U];
 #pragma unroll
         for (int k = 0; k < U; k++)
-          old[k] = ok[k] ? atomicAdd((unsigned long long*)&w.vs[x[k]].acc, (unsigned long long)rr[k])
+          old[k] = ok[k] ? atomicAdd((unsigned long long*)&w.vs[xa[k]].acc, (unsigned long long)ra[k])
                          : 1ull;
 #pragma unroll
         for (int k = 0; k < U; k++)
-          if (old[k] == 0) w.front[atomicAdd(&L->nfront, 1u)] = x[k];
+          if (okp[k] && op[k] == 0) w.front[atomicAdd(&L->nfront, 1u)] = xp[k];
+#pragma unroll
+        for (int k = 0; k < U; k++) {
+          xp[k] = xa[k];
+          op[k] = old[k];
+          okp[k] = ok[k];
+          xa[k] = xn[k];
+          ra[k] = rn[k];
+        }
       }
+#pragma unroll
+      for (int k = 0; k < U; k++)
+        if (okp[k] && op[k] == 0) w.front[atomicAdd(&L->nfront, 1u)] = xp[k];
     }
     w.edges += tot;
   }
@@ -1251,14 +1273,21 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
           if (!ok) dom = false;
         }
       } else {
-        for (u32 j = b; j < e; j++) {
-          u32 y = w.adj[j];
-          ed++;
-          if (w.vs[y].st == ST_X) continue;
-          if (y != us && !w.nmark[y]) {
-            dom = false;
-            break;
-          }
+        // four entries per step with their status and mark gathers in
+        // flight together; a padding entry of us always passes
+        for (u32 j = b; j < e && dom; j += 4) {
+          u32 y[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) y[t] = j + t < e ? w.adj[j + t] : us;
+          u8 st[4], nm[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) st[t] = w.vs[y[t]].st;
+#pragma unroll
+          for (int t = 0; t < 4; t++) nm[t] = w.nmark[y[t]];
+          ed += min(4u, e - j);
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+            if (st[t] != ST_X && y[t] != us && !nm[t]) dom = false;
         }
       }
       if (dom) w.work[atomicAdd(&L->ncand, 1u)] = x;
