@@ -389,9 +389,10 @@ __global__ void __launch_bounds__(256) ms_scatter(u32 n, u32 G, const u64* __res
 }
 
 // One adjacency unit (at most MS_UNIT entries): lane i loads entries i,
-// i+64, i+128, i+192 (four loads, then four gathers in flight) and keeps
-// w[t] = the neighbour id (0 past the unit's end) and mw[t] = the queries
-// that keep that neighbour.
+// i+64, i+128, i+192, then their vis words, and keeps w[t] = the neighbour
+// id (0 past the unit's end) and mw[t] = the queries that keep that
+// neighbour.  (Unconditional loads from clamped indices, which let all
+// eight be in flight, measured no faster: most units have one chunk.)
 __device__ __forceinline__ void ms_unit_load(const int32_t* __restrict__ adj,
                                              const u64* __restrict__ vis, i64 b0, u32 len,
                                              u32 w[4], u64 mw[4], u64& nbytes) {
