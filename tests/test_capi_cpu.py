@@ -71,3 +71,34 @@ def test_host_heuristic_equals_oracle(lib, gname):
         b = og.heuristic(0, l, h, q)
         assert a[0] == b[0], (gname, l, h, q)
         assert list(a[1]) == list(b[1]), (gname, l, h, q)
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """The Python binding's structures have the size and field offsets the C
+    header gives (a program built with g++ against include/scs.h prints
+    them), so a field added on one side only fails here."""
+    import subprocess
+    from scs_amd import scs
+    structs = {"scs_params": scs.Params, "scs_result": scs.Result, "scs_stats": scs.Stats,
+               "scs_model": scs.Model}
+    lines = ['#include <cstddef>', '#include <cstdio>', '#include "scs.h"', "int main() {"]
+    for cname, py in structs.items():
+        lines.append(f'  std::printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  std::printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.cpp"
+    src.write_text("\n".join(lines))
+    exe = str(tmp_path / "layout")
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", exe],
+                   check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            a, b, c = ln.split()
+            got[(a, b)] = int(c)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
