@@ -1186,15 +1186,16 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   };
   {
     u32 i = LANE;
-    // four candidates per lane per step: their gathers overlap
-    for (; i + 192 < nf; i += 256) {
-      const u32 x0 = w.front[i], x1 = w.front[i + 64], x2 = w.front[i + 128],
-                x3 = w.front[i + 192];
-      const VState v0 = w.vs[x0], v1 = w.vs[x1], v2 = w.vs[x2], v3 = w.vs[x3];
-      consider(x0, v0.acc + rterm(v0.dR, w.dmax));
-      consider(x1, v1.acc + rterm(v1.dR, w.dmax));
-      consider(x2, v2.acc + rterm(v2.dR, w.dmax));
-      consider(x3, v3.acc + rterm(v3.dR, w.dmax));
+    // eight candidates per lane per step: their gathers overlap
+    for (; i + 448 < nf; i += 512) {
+      u32 xs[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) xs[k] = w.front[i + 64 * k];
+      VState vv[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) vv[k] = w.vs[xs[k]];
+#pragma unroll
+      for (int k = 0; k < 8; k++) consider(xs[k], vv[k].acc + rterm(vv[k].dR, w.dmax));
     }
     for (; i < nf; i += 64) {
       const u32 x = w.front[i];
@@ -1260,16 +1261,16 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
       bool dom = true;
       const u32 b = w.off[x], e = w.off[x + 1];
       if (bm) {
-        // four entries per step (one cache line mostly); us is in tb, so a
-        // padding entry of us always passes
-        for (u32 j = b; j < e && dom; j += 4) {
-          const u32 y0 = w.adj[j];
-          const u32 y1 = j + 1 < e ? w.adj[j + 1] : us;
-          const u32 y2 = j + 2 < e ? w.adj[j + 2] : us;
-          const u32 y3 = j + 3 < e ? w.adj[j + 3] : us;
-          ed += min(4u, e - j);
-          const bool ok = (tbit(w, y0) || xbit(w, y0)) && (tbit(w, y1) || xbit(w, y1)) &&
-                          (tbit(w, y2) || xbit(w, y2)) && (tbit(w, y3) || xbit(w, y3));
+        // eight entries per step (their loads in flight together); us is
+        // in tb, so a padding entry of us always passes
+        for (u32 j = b; j < e && dom; j += 8) {
+          u32 y[8];
+#pragma unroll
+          for (int t = 0; t < 8; t++) y[t] = j + t < e ? w.adj[j + t] : us;
+          ed += min(8u, e - j);
+          bool ok = true;
+#pragma unroll
+          for (int t = 0; t < 8; t++) ok = ok && (tbit(w, y[t]) || xbit(w, y[t]));
           if (!ok) dom = false;
         }
       } else {
