@@ -769,7 +769,34 @@ __device__ __forceinline__ bool r12(W& w, u32 from, bool full, u32 kl, u64& tp, 
       atomicAdd((unsigned long long*)&w.gl->full_scans, 1ull);
       atomicAdd((unsigned long long*)&w.gl->full_rows, (unsigned long long)w.nR);
     }
-    for (u32 i = LANE; i < w.nR; i += 64) try_push(w, w.rlist[i], Cs, ubD, kl, w.work, &L->cnt);
+    // eight R vertices per lane per step: the list loads, then their
+    // state words and distances, are in flight together (indices past the
+    // list's end read its last entry and are ignored)
+    const u32 nR = w.nR;
+    const bool r1 = w.flags & QF_RED1, r2 = w.flags & QF_RED2;
+    for (u32 i0 = 0; i0 < nR; i0 += 512) {
+      u32 x[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) x[k] = w.rlist[min(i0 + LANE + 64u * k, nR - 1)];
+      u64 wd[8];
+      u32 ds[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) wd[k] = *(const u64*)&w.vs[x[k]];
+#pragma unroll
+      for (int k = 0; k < 8; k++) ds[k] = r2 ? w.dist[x[k]] : 0u;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const u32 dr = (u32)wd[k], dc = (u32)(wd[k] >> 32) & 0xffffu;
+        const u32 st = (u32)(wd[k] >> 48) & 0xffu, qd = (u32)(wd[k] >> 56);
+        const bool hit = (r1 && min(dr + dc, dc + w.h - Cs - 1) <= kl) || (r2 && ubD < ds[k]);
+        if (i0 + LANE + 64u * k < nR && st == ST_R && !qd && hit) {
+          u32* wp = (u32*)((uintptr_t)(&w.vs[x[k]].queued) & ~(uintptr_t)3);
+          const u32 sh = ((uintptr_t)(&w.vs[x[k]].queued) & 3) * 8;
+          const u32 old = atomicOr(wp, 1u << sh);
+          if (!((old >> sh) & 0xffu)) w.work[atomicAdd(&L->cnt, 1u)] = x[k];
+        }
+      }
+    }
   } else {
     // vertices removed since `from` (trail entries without the move or
     // batch-marker flags), compacted into work2, then their neighbours
