@@ -174,6 +174,26 @@ extern "C" int scs_graph_create(const int64_t* off, const int32_t* adj, uint32_t
     if (off[v + 1] < off[v]) return SCS_EINVAL;
   for (u64 j = 0; j < nnz; j++)
     if (adj[j] < 0 || (u32)adj[j] >= n) return SCS_EINVAL;
+  // The search kernels need a simple undirected graph: every row strictly
+  // ascending (sorted, no duplicates), no self-loops, and w in row v exactly
+  // when v is in row w.  Rows are visited in ascending v, so the entries
+  // below w in row w must match, in order, the rows v < w that name w.
+  {
+    std::vector<i64> ptr(off, off + n);
+    for (u32 v = 0; v < n; v++) {
+      for (i64 j = off[v]; j < off[v + 1]; j++) {
+        const u32 w = (u32)adj[j];
+        if (w == v) return SCS_EINVAL;
+        if (j > off[v] && adj[j - 1] >= adj[j]) return SCS_EINVAL;
+        if (w > v) {
+          if (ptr[w] >= off[w + 1] || (u32)adj[ptr[w]] != v) return SCS_EINVAL;
+          ptr[w]++;
+        }
+      }
+      // every entry of row v below v must have been matched by now
+      if (ptr[v] < off[v + 1] && (u32)adj[ptr[v]] < v) return SCS_EINVAL;
+    }
+  }
   scs_graph* G = new scs_graph;
   G->hg.n = n;
   G->hg.off.assign(off, off + n + 1);
@@ -343,7 +363,7 @@ struct scs_engine {
   std::deque<PendingQuery> queue;
   std::vector<HostSlot> slots;
   std::vector<QuerySlot> h_slots;
-  std::vector<scs_result> results;
+  std::deque<scs_result> results;
   std::vector<scs::HeuWork> heu_work;
   u32 active = 0;
   scs_stats st{};
@@ -453,6 +473,10 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   nslots = std::max<u32>(nslots, donate_below + p.limit_queries + 64);
   u32 slot_words = 4 + SCS_HCAP + (E->kcap + 1) / 2 + 64;
   if (p.buffer_size) nslots = std::max<u32>(p.limit_queries + 64, p.buffer_size / slot_words);
+  // limit_queries slots stay reserved for root tasks: a donation is refused
+  // once donor_cap tasks are pooled, so a root push never finds the pool full
+  const u32 donor_cap = nslots - p.limit_queries;
+  donate_below = std::min(donate_below, donor_cap);
   const u32 nq = p.limit_queries;
   // allocations
   if (hipMalloc(&E->d_slots, sizeof(QuerySlot) * nq) != hipSuccess) return fail(SCS_ENOMEM);
@@ -551,6 +575,7 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   A.glob = E->d_glob;
   A.slice_ticks = (u64)p.time_slice_us * 100ull;  // wall_clock64 runs at 100 MHz
   A.donate_below = donate_below;
+  A.donor_cap = donor_cap;
   A.deterministic = p.deterministic;
   A.node_flush = p.deterministic ? 1 : 16;
   A.waves_per_cu = p.deterministic ? 1 : p.waves_per_cu;
@@ -906,7 +931,10 @@ static int collect(scs_engine* E) {
     u32 ws = std::min<u32>(Q.wsize, SCS_MAX_H);
     if (ws) memcpy(wit.data(), E->h_wit + (u64)si * SCS_MAX_H, sizeof(u32) * ws);
     int status;
-    if (Q.capped)
+    // kl is raised only after its witness is stored, so kl == wkl whenever
+    // a witness exists; a lost witness (lock never acquired) leaves kl at the
+    // last stored one and the result is only a lower bound
+    if (Q.capped || Q.wlost || (Q.wsize && Q.wkl != Q.kl))
       status = SCS_ST_CAPPED;
     else if (Q.kl == 0 && hs.pq.l >= 2)
       status = SCS_ST_INFEASIBLE;
@@ -1105,12 +1133,28 @@ extern "C" int scs_step(scs_engine* E) {
     HIPCHK(hipMemcpyAsync(failed.data(), E->d_admit + E->p.limit_queries,
                           sizeof(u32) * admitted.size(), hipMemcpyDeviceToHost, E->s));
     HIPCHK(hipStreamSynchronize(E->s));
-    for (u32 f : failed)
-      if (f) {
-        fprintf(stderr, "scs: task pool full at admission\n");
-        return SCS_ENOMEM;
+    // a root task that found the pool full (not expected: donations leave
+    // limit_queries slots free) frees its slot and goes back to the queue
+    std::vector<PendingQuery> back;
+    u32 nok = 0;
+    for (u32 i = 0; i < na; i++) {
+      const u32 si = admitted[i];
+      if (!failed[i]) {
+        nok++;
+        continue;
       }
-    E->active += (u32)admitted.size();
+      QuerySlot& Q = E->h_slots[si];
+      memset(&Q, 0, sizeof(Q));
+      Q.state = QS_FREE;
+      HIPCHK(hipMemcpy(E->d_slots + si, &Q, sizeof(QuerySlot), hipMemcpyHostToDevice));
+      back.push_back(E->slots[si].pq);
+      release_slot(E, si);
+    }
+    if (!back.empty()) {
+      std::lock_guard<std::mutex> lk(E->qmu);
+      for (auto it = back.rbegin(); it != back.rend(); ++it) E->queue.push_front(*it);
+    }
+    E->active += nok;
   }
   size_t queued;
   {
@@ -1125,7 +1169,7 @@ extern "C" int scs_poll(scs_engine* E, scs_result* out, int max) {
   int c = 0;
   while (c < max && !E->results.empty()) {
     out[c++] = E->results.front();
-    E->results.erase(E->results.begin());
+    E->results.pop_front();
   }
   return c;
 }

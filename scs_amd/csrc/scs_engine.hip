@@ -682,23 +682,30 @@ __device__ void undo_to(W& w, u32 mark) {
 // Witness and kl.
 // ---------------------------------------------------------------------------
 __device__ void offer(W& w, u32 mn) {
+  // The witness is stored under the slot's lock before kl is raised, so a
+  // reader that sees kl >= x always finds a stored witness of min degree >= x.
   QuerySlot* Q = w.Q;
-  u32 improved = 0;
+  u32 want = 0;
   if (LANE == 0) {
-    u32 old = atomicMax(&Q->kl, mn);
-    if (old < mn) improved = 1;
-    else if (old == mn && aload(&Q->wsize) == 0) improved = 1;
+    const u32 cur = aload(&Q->kl);
+    want = (mn > cur || (mn == cur && aload(&Q->wsize) == 0)) ? 1u : 0u;
   }
-  improved = bcast(improved);
-  if (!improved) return;
+  want = bcast(want);
+  if (!want) return;
   u32 got = 0;
   if (LANE == 0) {
-    for (u32 spin = 0; spin < (1u << 20); spin++) {
+    // The holder's critical section is at most h stores; the bound only
+    // guards against a lost lock word.
+    for (u32 spin = 0; spin < (1u << 26); spin++) {
       if (atomicCAS(&Q->wlock, 0u, 1u) == 0u) {
         got = 1;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
+    }
+    if (!got) {
+      atomicExch(&Q->wlost, 1u);
+      atomicAdd(&w.gl->lock_fail, 1u);
     }
   }
   got = bcast(got);
@@ -713,6 +720,8 @@ __device__ void offer(W& w, u32 mn) {
     if (LANE == 0) {
       Q->wkl = mn;
       Q->wsize = w.nC;
+      __threadfence();
+      atomicMax(&Q->kl, mn);
     }
   }
   __threadfence();
@@ -1371,9 +1380,13 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
 // ---------------------------------------------------------------------------
 // Task pool (Vyukov bounded MPMC ring).  Lane 0 claims; the wave copies.
 // ---------------------------------------------------------------------------
-__device__ int pool_try_enqueue(const Pool& P, u64* out_pos) {
+// limit: most tasks the pool may hold after this enqueue.  deq only grows,
+// so pos - (deq read before the attempt) bounds the occupancy strictly.
+__device__ int pool_try_enqueue(const Pool& P, u64* out_pos, u32 limit) {
+  const u64 deq0 = aload64(P.deq);
   u64 pos = aload64(P.enq);
   for (int tries = 0; tries < 64; tries++) {
+    if (pos >= deq0 + limit) return 0;
     u64 s = aload64(&P.seq[pos % P.nslots]);
     long long dif = (long long)s - (long long)pos;
     if (dif == 0) {
@@ -1817,7 +1830,7 @@ __global__ void __launch_bounds__(64, 2) __attribute__((flatten)) scs_epoch(Engi
           const u32 need = 4 + f.csize + (form ? nRz : nXz);
           u64 pos = 0;
           u32 got = 0;
-          if (need <= A.pool.slot_words && LANE == 0) got = pool_try_enqueue(A.pool, &pos);
+          if (need <= A.pool.slot_words && LANE == 0) got = pool_try_enqueue(A.pool, &pos, A.donor_cap);
           got = bcast(got);
           if (got) {
             u32 plo = __shfl((u32)pos, 0), phi = __shfl((u32)(pos >> 32), 0);
@@ -1884,7 +1897,7 @@ __global__ void __launch_bounds__(64) scs_admit(EngineArgs A, const u32* slots_i
     const u32 sl = slots_in[i];
     u64 pos = 0;
     u32 got = 0;
-    if (LANE == 0) got = pool_try_enqueue(A.pool, &pos);
+    if (LANE == 0) got = pool_try_enqueue(A.pool, &pos, A.pool.nslots);
     got = bcast(got);
     if (!got) {
       if (LANE == 0) failed[i] = 1;

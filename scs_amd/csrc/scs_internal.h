@@ -58,7 +58,7 @@ struct QuerySlot {
   const u32* core; // core numbers (MLP feature)
   u32* witness;    // h entries, original ids
   u32 tag;         // low 32 bits of the caller's tag (search traces)
-  u32 pad;
+  u32 wlost;       // 1 = a better witness could not be stored (kl not raised for it)
 };
 
 // Depth-first frame: one per branching node on the current path.
@@ -200,6 +200,7 @@ struct EngineArgs {
   EngineGlobals* glob;
   u64 slice_ticks;     // wall_clock64 ticks per epoch
   u32 donate_below;    // donate while fewer tasks are pooled
+  u32 donor_cap;       // pooled tasks above which a donation is refused (root tasks keep the rest)
   u32 deterministic;
   u32 node_flush;      // flush node counters every this many nodes
   u32 waves_per_cu;    // resident waves per CU the launch is sized for

@@ -102,3 +102,43 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert got[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def _csr(n, rows):
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([len(r) for r in rows])
+    adj = np.array([x for r in rows for x in r], np.int32)
+    return off, adj
+
+
+@pytest.mark.parametrize("case,rows", [
+    ("asymmetric", [[1], [], [0]]),            # 0-1 without 1-0; 2-0 without 0-2
+    ("self_loop", [[0, 1], [0], []]),
+    ("duplicate", [[1, 1], [0, 0], []]),
+    ("unsorted", [[2, 1], [0], [0]]),
+    ("missing_reverse", [[1, 2], [0], []]),
+    ("out_of_range", [[3], [], []]),
+])
+def test_graph_create_rejects_non_simple_csr(case, rows):
+    """scs_graph_create validates the CSR on the host before any device call:
+    sorted unique rows, no self-loops, symmetric adjacency.  The search
+    kernels' per-neighbour updates assume exactly that."""
+    from scs_amd import scs
+    off, adj = _csr(3, rows)
+    with pytest.raises(scs.SCSError) as e:
+        scs.Graph.from_csr(off, adj)
+    assert e.value.code == -1, case
+
+
+def test_reference_loader_output_passes_validation(lib):
+    """A CSR that the reference-semantics loader produces (here the oracle's
+    restatement of it) is simple and symmetric, so validation accepts its
+    structure: the call goes on to the device and fails there only for lack
+    of a GPU, never with SCS_EINVAL."""
+    from scs_amd import scs
+    og = OracleGraph.load(os.path.join(GOLD, "hepPH_SCS"))
+    off, adj = og.csr()
+    try:
+        scs.Graph.from_csr(off, adj).close()
+    except scs.SCSError as e:
+        assert e.code != -1

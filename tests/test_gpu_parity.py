@@ -212,14 +212,16 @@ def test_parallel_mode_optimum_matches_golden(graphs):
                 check_witness(off, adj, q[2], q[0], q[1], got.kl, got.witness)
 
 
-def test_alone_equals_mix(graphs):
-    G, og = graphs["hepPH_SCS"]
-    rows = [r for r in GOLDEN["hepPH_SCS"]] + [r for r in GOLDEN["100.txt"]][:0]
-    qs = [tuple(r["query"]) for r in rows]
+@pytest.mark.parametrize("name", ["hepPH_SCS", "100.txt", "C1"])
+def test_alone_equals_mix(graphs, name):
+    """Every golden query of a graph gives the same optimum and status when
+    it shares the device with all the others as when it runs alone."""
+    G, og = graphs[name]
+    qs = [tuple(r["query"]) for r in GOLDEN[name]]
     mix, _ = run_engine(G, qs, limit_queries=64)
     for q, m in zip(qs, mix):
         alone, _ = run_engine(G, [q], limit_queries=1)
-        assert alone[0].kl == m.kl and alone[0].status == m.status
+        assert alone[0].kl == m.kl and alone[0].status == m.status, (name, q)
 
 
 def test_error_paths(graphs):
