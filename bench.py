@@ -35,7 +35,7 @@ BYTES_PER_EDGE = 4 + 7 + 6
 
 
 PHASES = ("rebuild", "rules", "bounds", "score", "doms", "move", "donate", "idle", "closure_scan",
-          "closure_remove", "argmax", "clear")
+          "closure_remove", "argmax", "clear", "pair")
 
 
 def phase_shares(s0, s1):

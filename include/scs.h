@@ -100,7 +100,14 @@ typedef struct scs_params {
   uint32_t no_lds_bitmaps; /* 1 = keep vertex status in device memory only, even for
                               kept sets whose bitmaps fit in LDS (the path large kept
                               sets take; for testing it on small graphs) */
-  uint32_t reserved[5];
+  uint32_t levels;         /* 1 = descending level floors: search for min degree
+                              >= ku first, then lower bounds down to the heuristic's
+                              kl (same optimum, no counterpart in the reference) */
+  uint32_t pair;           /* 1 = common-neighbour pair rule: two vertices of an H
+                              with min degree > kl and |H| <= h share at least
+                              2(kl+1)-h neighbours in H (adjacent) or 2(kl+1)-h+2
+                              (not adjacent); no counterpart in the reference */
+  uint32_t reserved[3];
 } scs_params;
 
 typedef struct scs_result {
@@ -133,6 +140,7 @@ int scs_graph_core(const scs_graph* g, uint32_t* core);
 int scs_graph_csr(const scs_graph* g, int64_t* off, int32_t* adj);
 /* Device time of the core decomposition in microseconds. */
 double scs_graph_core_time_us(const scs_graph* g);
+
 void scs_graph_destroy(scs_graph* g);
 
 void scs_params_default(scs_params* p);
@@ -168,10 +176,11 @@ typedef struct scs_stats {
   uint64_t edges;          /* adjacency entries scanned by the search */
   double core_us;          /* device time of the core decomposition */
   uint64_t prep_batches;   /* batched preprocessing passes (up to 64 queries each) */
-  uint64_t phase_ticks[12]; /* search wall-clock ticks (100 MHz) summed over waves:
+  uint64_t phase_ticks[13]; /* search wall-clock ticks (100 MHz) summed over waves:
                               rebuild, rules (other), bounds, score (scatter and
                               shortlist), doms, move, donate, idle, closure scan,
-                              closure removals, score argmax, score clearing */
+                              closure removals, score argmax, score clearing,
+                              pair rule */
   uint64_t dom_front;      /* frontier vertices summed over branching nodes */
   uint64_t dom_cand;       /* frontier vertices tested for domination */
   uint64_t dom_found;      /* tested vertices found dominated by ustar */

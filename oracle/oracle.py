@@ -18,6 +18,8 @@ _LIB = None
 
 FLAGS_ALL = 0x3F  # red1 red2 red3 prun1 prun2 doms
 FLAG_RECOMPUTE = 0x40  # use the recompute solver (same nodes as the trail solver)
+FLAG_LEVELS = 0x80  # descending level floors from ku-1 (same optimum, different search)
+FLAG_PAIR = 0x100  # common-neighbour pair rule (trail solver)
 
 STATUS = {0: "OK", 1: "HEU_OPT", 2: "CAPPED", 3: "INFEASIBLE", -1: "INVALID"}
 
@@ -25,7 +27,7 @@ STATUS = {0: "OK", 1: "HEU_OPT", 2: "CAPPED", 3: "INFEASIBLE", -1: "INVALID"}
 class OrcResult(C.Structure):
     _fields_ = [("kl", C.c_uint32), ("heu_kl", C.c_uint32), ("ku", C.c_uint32),
                 ("status", C.c_int32), ("nodes", C.c_uint64), ("K", C.c_uint32),
-                ("ubD", C.c_uint32), ("wsize", C.c_uint32), ("pad", C.c_uint32)]
+                ("ubD", C.c_uint32), ("wsize", C.c_uint32), ("levels", C.c_uint32)]
 
 
 def build() -> str:
@@ -155,7 +157,7 @@ class OracleGraph:
         w = np.zeros(wcap, np.uint32)
         lib().orc_solve(self.h, l, h, q, heu, limit_doms, flags, node_cap, C.byref(r), w, wcap)
         return dict(kl=r.kl, heu_kl=r.heu_kl, ku=r.ku, status=STATUS[r.status], nodes=r.nodes,
-                    K=r.K, ubD=r.ubD, witness=w[:min(r.wsize, wcap)].copy())
+                    K=r.K, ubD=r.ubD, levels=r.levels, witness=w[:min(r.wsize, wcap)].copy())
 
     def solve_plain(self, l, h, q, node_cap=1 << 40):
         nodes = C.c_uint64(0)

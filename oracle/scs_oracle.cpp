@@ -490,6 +490,7 @@ struct Params {
   u32 l, h, limit_doms;
   u32 red1, red2, red3, prun1, prun2;
   u32 use_doms;
+  u32 pair = 0;
   u64 node_cap;
   u32 dmax;
   u32 ku;
@@ -514,7 +515,13 @@ struct RecomputeSolver {
     std::vector<u32> C, R;
   };
 
-  RecomputeSolver(const Local& L_, const Params& P_, u32 kl0) : L(L_), P(P_), kl(kl0) {
+  u32 real = 0;  // min degree of the best witness (or the heuristic's kl0)
+  u32 flr = 0;   // level floor: rules and offers use kl = max(real, flr)
+  void set_floor(u32 f) {
+    flr = f;
+    kl = std::max(real, flr);
+  }
+  RecomputeSolver(const Local& L_, const Params& P_, u32 kl0) : L(L_), P(P_), kl(kl0), real(kl0) {
     st.assign(L.K, ST_X);
     dC.assign(L.K, 0);
     dR.assign(L.K, 0);
@@ -561,8 +568,9 @@ struct RecomputeSolver {
       if (C.size() >= P.l && C.size() <= h) {
         u32 mn = UINT32_MAX;
         for (u32 u : C) mn = std::min(mn, dC[u]);
-        if (mn > kl || (!have_witness && mn >= kl)) {
-          kl = std::max(kl, mn);
+        if (mn >= kl && (mn > real || (!have_witness && mn >= real))) {
+          real = mn;
+          kl = std::max(real, flr);
           witness = C;
           have_witness = true;
         }
@@ -711,7 +719,7 @@ struct RecomputeSolver {
     stack.push_back(std::move(root));
     std::vector<Task> kids;
     while (!stack.empty()) {
-      if (kl >= P.ku) return 0;
+      if (real >= P.ku) return 0;
       if (nodes >= P.node_cap) return 1;
       Task t = std::move(stack.back());
       stack.pop_back();
@@ -742,13 +750,22 @@ struct TrailSolver {
   };
   std::vector<Frame> frames;
   u32 ustar = 0;
+  std::vector<u32> pcnt;            // common-neighbour counts (pair rule)
+  std::vector<unsigned char> padj;  // 1 = neighbour of the C vertex being checked
+  std::vector<u32> ptouch, prl, prm;
   u32 closed_kl = 0;
   std::vector<std::pair<u64, u32>> Dv;
   std::vector<u32> front;
   std::vector<unsigned char> inF;
   std::vector<u64> acc, accd;
 
-  TrailSolver(const Local& L_, const Params& P_, u32 kl0) : L(L_), P(P_), kl(kl0) {}
+  u32 real = 0;  // min degree of the best witness (or the heuristic's kl0)
+  u32 flr = 0;   // level floor: rules and offers use kl = max(real, flr)
+  void set_floor(u32 f) {
+    flr = f;
+    kl = std::max(real, flr);
+  }
+  TrailSolver(const Local& L_, const Params& P_, u32 kl0) : L(L_), P(P_), kl(kl0), real(kl0) {}
 
   void moveC(u32 v) {
     st[v] = ST_C;
@@ -851,6 +868,64 @@ struct TrailSolver {
     return removed;
   }
 
+  // Pair rule.  Two vertices u, v of an H with min degree >= k = kl+1 and
+  // |H| <= h each have >= k - [u~v] neighbours in H \ {u, v}, a set of at
+  // most h-2 vertices, so they share >= 2k - h (adjacent) or 2k - h + 2 (not
+  // adjacent) neighbours in H, all of them live (C or R).  For each C vertex
+  // u in C order: count the live common neighbours of u with every live
+  // vertex (two-hop scatter over the live neighbours of u), prune the node
+  // if another C vertex falls short, else remove every R vertex that falls
+  // short before the next u.  Returns -1 prune, 1 when something was
+  // removed, 0 otherwise.
+  int pair_step() {
+    const long tadj = 2L * (long)(kl + 1) - (long)P.h;
+    if (tadj + 2 <= 0) return 0;
+    if (pcnt.size() != L.K) {
+      pcnt.assign(L.K, 0);
+      padj.assign(L.K, 0);
+    }
+    int any = 0;
+    prl.clear();
+    for (u32 v = 0; v < L.K; v++)
+      if (st[v] == ST_R) prl.push_back(v);
+    for (size_t ci = 0; ci < Clist.size(); ci++) {
+      const u32 u = Clist[ci];
+      ptouch.clear();
+      for (u32 j = L.off[u]; j < L.off[u + 1]; j++) padj[L.adj[j]] = 1;
+      for (u32 j = L.off[u]; j < L.off[u + 1]; j++) {
+        const u32 w = L.adj[j];
+        if (st[w] == ST_X) continue;
+        for (u32 t = L.off[w]; t < L.off[w + 1]; t++) {
+          const u32 x = L.adj[t];
+          if (st[x] == ST_X) continue;
+          if (pcnt[x]++ == 0) ptouch.push_back(x);
+        }
+      }
+      auto short_of = [&](u32 v) { return (long)pcnt[v] < (padj[v] ? tadj : tadj + 2); };
+      bool prune = false;
+      for (u32 c : Clist)
+        if (c != u && short_of(c)) prune = true;
+      prm.clear();
+      if (!prune) {
+        // R vertices in ascending order; the survivors stay in prl
+        size_t k = 0;
+        for (u32 v : prl) {
+          if (short_of(v))
+            prm.push_back(v);
+          else
+            prl[k++] = v;
+        }
+        prl.resize(k);
+      }
+      for (u32 x : ptouch) pcnt[x] = 0;
+      for (u32 j = L.off[u]; j < L.off[u + 1]; j++) padj[L.adj[j]] = 0;
+      if (prune) return -1;
+      for (u32 v : prm) removeR(v);
+      if (!prm.empty()) any = 1;
+    }
+    return any;
+  }
+
   // kl_seen: kl at the parent's closure (UINT32_MAX at the root)
   int process(u32 from, u32 kl_seen) {
     nodes++;
@@ -861,9 +936,11 @@ struct TrailSolver {
       if (Cs >= P.l && Cs <= h) {
         u32 mn = UINT32_MAX;
         for (u32 u : Clist) mn = std::min(mn, dC[u]);
-        if (mn > kl || (!have_witness && mn >= kl)) {
-          if (mn > kl) full = true;
-          kl = std::max(kl, mn);
+        if (mn >= kl && (mn > real || (!have_witness && mn >= real))) {
+          real = mn;
+          const u32 ne = std::max(real, flr);
+          if (ne != kl) full = true;
+          kl = ne;
           witness = Clist;
           have_witness = true;
         }
@@ -886,6 +963,11 @@ struct TrailSolver {
       from = (u32)trail.size();
       full = false;
       if (removed && P.red3 && tight_exists()) continue;
+      if (P.pair) {
+        const int pr = pair_step();
+        if (pr < 0) return NODE_PRUNE;
+        if (pr > 0) continue;
+      }
       break;
     }
     closed_kl = kl;
@@ -979,7 +1061,7 @@ struct TrailSolver {
     trail.clear();
     u32 from = 0, kl_seen = UINT32_MAX;
     while (true) {
-      if (kl >= P.ku) return 0;
+      if (real >= P.ku) return 0;
       if (nodes >= P.node_cap) return 1;
       int r = process(from, kl_seen);
       if (r == NODE_BRANCH) {
@@ -989,6 +1071,7 @@ struct TrailSolver {
         f.nD = (u32)Dv.size();
         f.next = 0;
         f.kl = closed_kl;
+
         for (u32 i = 0; i < f.nD; i++) f.D[i] = Dv[i].second;
         frames.push_back(f);
         from = f.mark;
@@ -1185,11 +1268,12 @@ struct orc_result {
   u32 K;        // kept vertices
   u32 ubD;
   u32 wsize;
-  u32 pad;
+  u32 levels;   // level searches run (1 without levels)
 };
 
 // flags bit0 red1, bit1 red2, bit2 red3, bit3 prun1, bit4 prun2, bit5 doms,
-// bit6 selects the recompute solver instead of the trail solver
+// bit6 selects the recompute solver instead of the trail solver, bit7 levels,
+// bit8 pair rule (trail solver only)
 int orc_solve(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 flags,
               u64 node_cap, orc_result* res, u32* wit, u32 wcap) {
   memset(res, 0, sizeof(*res));
@@ -1237,28 +1321,51 @@ int orc_solve(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 fl
   P.prun1 = (flags >> 3) & 1;
   P.prun2 = (flags >> 4) & 1;
   P.use_doms = (flags >> 5) & 1;
+  P.pair = (flags >> 8) & 1;
   P.node_cap = node_cap;
   P.dmax = g.dmax;
   P.ku = ku;
-  u32 final_kl;
+  // Levels (flag bit 7): search for min degree > flr with flr = ku-1 first,
+  // then lower floors, down to the heuristic's kl.  A level whose search
+  // ends without a solution above its floor proves opt <= flr; the first
+  // level that finds one returns the optimum, as its search was exact for
+  // every solution above its floor.  A level that ends empty also lowers the
+  // stopping bound ku to its floor (opt <= flr), so the next level stops as
+  // soon as it finds a solution at that bound.
+  const bool levels = (flags >> 7) & 1;
+  u32 flr = (levels && ku > kl + 1) ? ku - 1 : 0;
+  u32 real = kl;
+  bool have_w = kl > 0;
   std::vector<u32> wloc;
-  int capped;
-  u64 nodes;
-  if ((flags >> 6) & 1) {
-    RecomputeSolver S(L, P, kl);
-    if (kl > 0) S.have_witness = true;
-    capped = S.run(L.qloc);
-    final_kl = S.kl;
-    wloc = S.witness;
-    nodes = S.nodes;
-  } else {
-    TrailSolver S(L, P, kl);
-    if (kl > 0) S.have_witness = true;
-    capped = S.run(L.qloc);
-    final_kl = S.kl;
-    wloc = S.witness;
-    nodes = S.nodes;
+  int capped = 0;
+  u64 nodes = 0;
+  while (true) {
+    P.node_cap = node_cap > nodes ? node_cap - nodes : 0;
+    if ((flags >> 6) & 1) {
+      RecomputeSolver S(L, P, real);
+      S.have_witness = have_w;
+      S.set_floor(flr);
+      capped = S.run(L.qloc);
+      if (S.real > real || (!have_w && S.have_witness)) wloc = S.witness;
+      real = S.real;
+      have_w = S.have_witness;
+      nodes += S.nodes;
+    } else {
+      TrailSolver S(L, P, real);
+      S.have_witness = have_w;
+      S.set_floor(flr);
+      capped = S.run(L.qloc);
+      if (S.real > real || (!have_w && S.have_witness)) wloc = S.witness;
+      real = S.real;
+      have_w = S.have_witness;
+      nodes += S.nodes;
+    }
+    res->levels++;
+    if (capped || real >= flr) break;
+    P.ku = std::min(P.ku, flr);
+    flr--;
   }
+  const u32 final_kl = real;
   res->kl = final_kl;
   res->nodes = nodes;
   if (!wloc.empty()) {

@@ -15,6 +15,7 @@ C5  large combined: R-MAT seed 5, scale 22, edge factor 16; 20,000 queries,
 
 from __future__ import annotations
 
+import os
 from functools import lru_cache
 
 import numpy as np
@@ -42,12 +43,37 @@ CONFIGS = {
 }
 
 
-@lru_cache(maxsize=4)
-def graph(name: str) -> gg.CSR:
+def generate(name: str) -> gg.CSR:
+    """Builds the graph of a workload from its generator parameters."""
     c = CONFIGS[name]
     if c["kind"] == "chung_lu":
         return gg.chung_lu(c["n"], c["m"], c["beta"], c["dmax"], c["seed"])
     return gg.rmat(c["scale"], c["edge_factor"], c["a"], c["b"], c["c"], c["seed"])
+
+
+def graph_params(name: str) -> dict:
+    """The generator parameters that fix a workload's graph bit for bit."""
+    return {k: v for k, v in CONFIGS[name].items() if k != "queries"}
+
+
+@lru_cache(maxsize=4)
+def graph(name: str, cache_dir: str | None = None) -> gg.CSR:
+    """The workload's graph, from the binary CSR cache when it holds one for
+    these generator parameters, else generated (and then cached).  The cache
+    directory is ``cache_dir``, or $SCS_CSR_CACHE, or _cache/csr beside the
+    package; SCS_CSR_CACHE=off disables it."""
+    d = cache_dir or os.environ.get("SCS_CSR_CACHE") or os.path.join(
+        os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "_cache", "csr")
+    if d == "off":
+        return generate(name)
+    g = gg.cache_load(d, graph_params(name))
+    if g is None:
+        g = generate(name)
+        try:
+            gg.cache_store(d, graph_params(name), g)
+        except OSError:
+            pass  # read-only home: generation still works
+    return g
 
 
 def queries(name: str, core: np.ndarray, count: int | None = None) -> list[gg.Query]:

@@ -390,6 +390,8 @@ extern "C" void scs_params_default(scs_params* p) {
   p->model = nullptr;
   p->heuristic_threads = 8;
   p->admit_per_step = 256;
+  p->levels = 1;
+  p->pair = 1;
 }
 
 static u64 align256(u64 x) { return (x + 255) & ~255ull; }
@@ -735,13 +737,16 @@ static void fill_slot(scs_engine* E, u32 si, const MsQuery& P, u32 qloc) {
   const scs_params& p = E->p;
   Q.flags = (p.red1 ? QF_RED1 : 0) | (p.red2 ? QF_RED2 : 0) | (p.red3 ? QF_RED3 : 0) |
             (p.prun1 ? QF_PRUN1 : 0) | (p.prun2 ? QF_PRUN2 : 0) | QF_DOMS |
-            (p.learned ? QF_LEARNED : 0);
+            (p.learned ? QF_LEARNED : 0) | (p.levels ? QF_LEVELS : 0) | (p.pair ? QF_PAIR : 0);
   Q.tag = (u32)hs.pq.tag;
   Q.ku = hs.ku;
   Q.K = hs.K;
   Q.dmax = E->g->hg.dmax;
   Q.qloc = qloc;
   Q.kl = hs.kl0;
+  // first level: min degree >= ku (floor ku-1), when that is above kl0 + 1
+  Q.floor = (p.levels && hs.ku > hs.kl0 + 1) ? hs.ku - 1 : 0;
+  Q.lvl_nodes0 = 0;
   Q.pending = 1;
   Q.node_budget = p.node_budget;
   Q.off = P.loff;

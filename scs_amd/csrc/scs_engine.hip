@@ -124,6 +124,7 @@ struct Lds {
 #define PH_CREMOVE 9  // rule 1/2 closure: applying the removal batches
 #define PH_ARGMAX 10  // argmax of the connection score over the frontier
 #define PH_CLEAR 11   // clearing the frontier's scores
+#define PH_PAIR 12    // pair rule (common-neighbour counts of C vertices)
 
 __device__ __forceinline__ void ph_switch(Lds* L, u64& tp, u32& ph, u32 next) {
   const u64 t = wall_clock64();
@@ -729,7 +730,10 @@ __device__ void offer(W& w, u32 mn) {
   wsync();
 }
 
-__device__ u32 cur_kl(const W& w) { return aload(&w.Q->kl); }
+// The bound the rules work against: the best witness's min degree, or the
+// current level's floor when that is higher (no task of the query runs while
+// the floor changes).
+__device__ u32 cur_kl(const W& w) { return max(aload(&w.Q->kl), aload(&w.Q->floor)); }
 
 // ---------------------------------------------------------------------------
 // Node processing.
@@ -862,6 +866,90 @@ __device__ bool tight_exists(const W& w, u32 kl) {
     if (w.vs[u].dC + w.vs[u].dR == kl + 1 && w.vs[u].dR > 0) f = 1;
   }
   return __ballot(f) != 0;
+}
+
+// Pair rule.  Two vertices u, v of an H with min degree >= k = kl+1 and
+// |H| <= h each have >= k - [u~v] neighbours in H minus {u, v}, a set of at
+// most h-2 vertices, so they share >= 2k-h (adjacent) or 2k-h+2 (not
+// adjacent) neighbours in H, all live (C or R).  For each C vertex u in C
+// order: the live common neighbours of u with every live vertex are counted
+// by a two-hop scatter (low word of acc, which is zero between uses; first
+// touches listed in front, N(u) marked in nmark); the node is pruned when
+// another C vertex falls short, else every R vertex that falls short is
+// removed as one batch before the next u.  Returns -1 (prune), 1 (removed
+// something) or 0.
+__device__ __forceinline__ int pair_step(W& w, u32 kl) {
+  Lds* L = w.L;
+  const long tadj = 2L * (long)(kl + 1) - (long)w.h;
+  if (tadj + 2 <= 0) return 0;
+  const Clo none{0, 0, nullptr, nullptr};
+  int any = 0;
+  for (u32 ci = 0; ci < w.nC; ci++) {
+    const u32 u = w.clist[ci];
+    const u32 ub = w.off[u], ue = w.off[u + 1];
+    if (LANE == 0) {
+      L->nfront = 0;
+      L->cnt = 0;
+    }
+    u32 nl = 0;
+    for (u32 j0 = ub; j0 < ue; j0 += 64) {
+      const u32 j = j0 + LANE;
+      u32 x = 0;
+      bool live = false;
+      if (j < ue) {
+        x = w.adj[j];
+        w.nmark[x] = 1;
+        live = dec_live(w, x);
+      }
+      const u64 bmk = __ballot(live);
+      if (live) w.work2[nl + __popcll(bmk & lanes_below())] = x;
+      nl += __popcll(bmk);
+    }
+    w.edges += ue - ub;
+    wsync();
+    const u32* lv = w.work2;
+    scan_adj<u32>(
+        w, nl, [&](u32 i) { return lv[i]; },
+        [&](u32 x) {
+          if (dec_live(w, x)) {
+            const u32 old = atomicAdd((u32*)&w.vs[x].acc, 1u);
+            if (old == 0) w.front[atomicAdd(&L->nfront, 1u)] = x;
+          }
+          return 0u;
+        },
+        [&](u32, u32, u32) {});
+    // the barrier's fence waits for the count atomics
+    wsync();
+    auto short_of = [&](u32 v) -> bool {
+      const long c = (long)(u32)w.vs[v].acc;
+      return c < (w.nmark[v] ? tadj : tadj + 2);
+    };
+    u32 pr = 0;
+    for (u32 i = LANE; i < w.nC; i += 64) {
+      const u32 c = w.clist[i];
+      if (c != u && short_of(c)) pr = 1;
+    }
+    const bool prune = __ballot(pr) != 0;
+    if (!prune) {
+      const u32 nR = w.nR;
+      for (u32 i = LANE; i < nR; i += 64) {
+        const u32 v = w.rlist[i];
+        if (short_of(v)) w.work[atomicAdd(&L->cnt, 1u)] = v;
+      }
+    }
+    wsync();
+    const u32 nt = bcast(L->nfront);
+    for (u32 i = LANE; i < nt; i += 64) w.vs[w.front[i]].acc = 0;
+    for (u32 j = ub + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 0;
+    wsync();
+    if (prune) return -1;
+    const u32 m = bcast(L->cnt);
+    if (m) {
+      remove_batch(w, w.work, m, none);
+      any = 1;
+    }
+  }
+  return any;
 }
 
 // U_nr water-filling on one lane: C degrees (histogram) get +1 on the
@@ -1053,6 +1141,16 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     from = w.ntrail;
     full = false;
     if (removed && (w.flags & QF_RED3) && tight_exists(w, kl)) continue;
+    if (w.flags & QF_PAIR) {
+      ph_switch(L, tp, ph, PH_PAIR);
+      const int pr = pair_step(w, kl);
+      ph_switch(L, tp, ph, PH_RULES);
+      if (pr < 0) {
+        o.klc = kl;
+        return o;
+      }
+      if (pr > 0) continue;
+    }
     break;
   }
   o.klc = kl;
@@ -1602,14 +1700,46 @@ __device__ __forceinline__ void save_state(W& w, u32 active, u32 qslot, u32 from
   wsync();
 }
 
+// The wave that ends a query's last task either starts the query's next
+// level or marks the query done.  A level that ends with no witness above
+// its floor proves opt <= floor: the stopping bound ku drops to the floor
+// and the root task (C = {q}) goes back to the pool with the floor one
+// lower.  Every task of the query has ended and flushed its node count, so
+// the slot's fields are this wave's alone here.
 __device__ void finish_task(W& w, const EngineArgs& A, u32 qslot) {
   if (LANE == 0) {
     QuerySlot* Q = &A.slots[qslot];
     u32 left = atomicSub(&Q->pending, 1u) - 1;
     if (left == 0) {
       __threadfence();
-      atomicExch(&Q->state, (u32)QS_DONE);
-      atomicSub(&A.glob->active_q, 1u);
+      const u32 kl = aload(&Q->kl), fl = aload(&Q->floor);
+      bool again = false;
+      if ((Q->flags & QF_LEVELS) && !aload(&Q->stop) && kl < fl) {
+        u64 pos = 0;
+        if (pool_try_enqueue(A.pool, &pos, A.pool.nslots)) {
+          Q->ku = min(aload(&Q->ku), fl);
+          Q->floor = fl - 1;
+          Q->lvl_nodes0 = aload64(&Q->nodes);
+          Q->pending = 1;
+          u32* dst = A.pool.data + (pos % A.pool.nslots) * (u64)A.pool.slot_words;
+          dst[0] = qslot;
+          dst[1] = 1;
+          dst[2] = 0;
+          dst[3] = 0;
+          dst[4] = Q->qloc;
+          __threadfence();
+          astore64(&A.pool.seq[pos % A.pool.nslots], pos + 1);
+          again = true;
+        } else {
+          // no pool slot (root slots are reserved, so not expected): the
+          // result is reported as a lower bound
+          atomicExch(&Q->capped, 1u);
+        }
+      }
+      if (!again) {
+        atomicExch(&Q->state, (u32)QS_DONE);
+        atomicSub(&A.glob->active_q, 1u);
+      }
     }
   }
   wsync();
@@ -1724,7 +1854,7 @@ __global__ void __launch_bounds__(64, 2) __attribute__((flatten)) scs_epoch(Engi
     if (LANE == 0) {
       if (aload(&Q->stop)) {
         quit = 1;
-      } else if (aload(&Q->kl) >= Q->ku) {
+      } else if (aload(&Q->kl) >= aload(&Q->ku)) {
         atomicExch(&Q->stop, 1u);
         quit = 1;
       } else if (Q->node_budget && aload64(&Q->nodes) + pend >= Q->node_budget) {

@@ -14,7 +14,7 @@ typedef int64_t i64;
 #define SCS_WAVE 64
 #define SCS_FRAME_DOMS SCS_MAX_DOMS
 #define SCS_HCAP (SCS_MAX_H + 2)
-#define SCS_PHASES 12
+#define SCS_PHASES 13
 
 // vertex status inside a search state
 #define ST_R 0
@@ -34,6 +34,8 @@ typedef int64_t i64;
 #define QF_PRUN2 16u
 #define QF_DOMS 32u
 #define QF_LEARNED 64u
+#define QF_LEVELS 128u  // descending level floors
+#define QF_PAIR 256u    // common-neighbour pair rule
 
 // Per-query slot, device resident.  Written by the host at admission (while
 // no kernel touches the slot) and by search waves through atomics.
@@ -59,6 +61,9 @@ struct QuerySlot {
   u32* witness;    // h entries, original ids
   u32 tag;         // low 32 bits of the caller's tag (search traces)
   u32 wlost;       // 1 = a better witness could not be stored (kl not raised for it)
+  u32 floor;       // level floor: the search looks for min degree > max(kl, floor)
+  u32 pad2;
+  u64 lvl_nodes0;  // nodes when this level started
 };
 
 // Depth-first frame: one per branching node on the current path.

@@ -237,6 +237,50 @@ def make_queries(core: np.ndarray, count: int, seed: int, *, core_min: int,
     return out
 
 
+# Binary CSR cache.  One directory entry per parameter set: <key>.npz holds
+# n, off (int64), adj (int32) and the CSR's SHA-256; the key is the SHA-256
+# of the generator parameters (sorted JSON) and CACHE_VERSION, so a change of
+# generator code that changes graphs only needs a new version string.
+CACHE_VERSION = "scs-csr-1"
+
+
+def cache_key(params: dict) -> str:
+    import json
+    blob = json.dumps({"v": CACHE_VERSION, **params}, sort_keys=True).encode()
+    return hashlib.sha256(blob).hexdigest()[:32]
+
+
+def cache_store(directory: str, params: dict, g: CSR) -> str:
+    """Writes g under the parameters' key (atomically: temp file, rename)."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, cache_key(params) + ".npz")
+    tmp = path + f".tmp{os.getpid()}.npz"
+    np.savez(tmp, n=np.int64(g.n), off=np.ascontiguousarray(g.off, np.int64),
+             adj=np.ascontiguousarray(g.adj, np.int32),
+             sha256=np.frombuffer(bytes.fromhex(g.sha256()), np.uint8))
+    os.replace(tmp, path)
+    return path
+
+
+def cache_load(directory: str, params: dict, verify: bool = True) -> CSR | None:
+    """The cached CSR of these parameters, or None when absent or when its
+    stored SHA-256 does not match its contents (the entry is then ignored)."""
+    import os
+    path = os.path.join(directory, cache_key(params) + ".npz")
+    if not os.path.exists(path):
+        return None
+    try:
+        with np.load(path, allow_pickle=False) as z:
+            g = CSR(n=int(z["n"]), off=z["off"], adj=z["adj"])
+            want = bytes(z["sha256"]).hex()
+    except (OSError, ValueError, KeyError):
+        return None
+    if verify and g.sha256() != want:
+        return None
+    return g
+
+
 def write_edge_list(g: CSR, path: str) -> None:
     """Writes the reference's edge-list format: header ``n m`` then one
     tab-separated undirected edge per line (src/Graph.h:16-23)."""

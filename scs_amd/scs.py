@@ -46,7 +46,7 @@ class Params(C.Structure):
                 ("deterministic", C.c_uint32), ("learned", C.c_uint32),
                 ("model", C.c_void_p), ("heuristic_threads", C.c_uint32),
                 ("admit_per_step", C.c_uint32), ("no_lds_bitmaps", C.c_uint32),
-                ("reserved", C.c_uint32 * 5)]
+                ("levels", C.c_uint32), ("pair", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class Result(C.Structure):
@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("engine_us", C.c_double), ("prep_us", C.c_double), ("heu_us", C.c_double),
                 ("prep_bytes", C.c_uint64), ("epochs", C.c_uint64), ("waves", C.c_uint32),
                 ("kcap", C.c_uint32), ("edges", C.c_uint64), ("core_us", C.c_double),
-                ("prep_batches", C.c_uint64), ("phase_ticks", C.c_uint64 * 12),
+                ("prep_batches", C.c_uint64), ("phase_ticks", C.c_uint64 * 13),
                 ("dom_front", C.c_uint64), ("dom_cand", C.c_uint64), ("dom_found", C.c_uint64),
                 ("full_scans", C.c_uint64), ("full_rows", C.c_uint64)]
 

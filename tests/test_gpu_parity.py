@@ -16,7 +16,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle.oracle import OracleGraph, ubd_host
+from oracle.oracle import FLAG_LEVELS, FLAG_PAIR, FLAGS_ALL, OracleGraph, ubd_host
 from scs_amd import configs, scs
 from tests.helpers import check_witness
 
@@ -164,12 +164,20 @@ def run_engine(G, queries, **kw):
     return res, st
 
 
+# search modes: the reference's rules alone; with descending level floors;
+# with level floors and the pair rule (the engine's default)
+MODES = {"ref": (0, 0, FLAGS_ALL), "levels": (1, 0, FLAGS_ALL | FLAG_LEVELS),
+         "levels_pair": (1, 1, FLAGS_ALL | FLAG_LEVELS | FLAG_PAIR)}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("no_bitmaps", [0, 1])
 @pytest.mark.parametrize("name", ["graph.txt", "100.txt", "hepPH_SCS", "C1"])
-def test_deterministic_mode_expands_the_oracles_nodes(graphs, name, no_bitmaps):
+def test_deterministic_mode_expands_the_oracles_nodes(graphs, name, no_bitmaps, mode):
     """no_bitmaps=1 runs the device-memory status path that kept sets too
     large for the LDS bitmaps take."""
     G, og = graphs[name]
+    levels, pair, flags = MODES[mode]
     if name == "C1":
         core = og.core()
         qs = [(q.l, q.h, q.q, q.heu, q.limit_doms)
@@ -178,11 +186,34 @@ def test_deterministic_mode_expands_the_oracles_nodes(graphs, name, no_bitmaps):
     else:
         qs = [tuple(r["query"]) for r in GOLDEN[name] if r["nodes"] <= 20000]
         cap = 20000
+    print(name, mode, "no_bitmaps", no_bitmaps, flush=True)
     res, _ = run_engine(G, qs, deterministic=1, node_budget=cap, limit_queries=16,
-                        no_lds_bitmaps=no_bitmaps)
+                        no_lds_bitmaps=no_bitmaps, levels=levels, pair=pair)
     for r, q in zip(res, qs):
-        o = og.solve(*q, node_cap=cap)
+        o = og.solve(*q, flags=flags, node_cap=cap)
         assert (r.kl, r.status, r.nodes) == (o["kl"], o["status"], o["nodes"]), (name, q)
+
+
+# the reference's rule and bound switches (CLI red1 red2 red3 prun1 prun2,
+# main.cu:481-502): each one off, and all of them off
+SWITCHES = {"red1": (0, 1, 1, 1, 1), "red2": (1, 0, 1, 1, 1), "red3": (1, 1, 0, 1, 1),
+            "prun1": (1, 1, 1, 0, 1), "prun2": (1, 1, 1, 1, 0), "none": (0, 0, 0, 0, 0)}
+
+
+@pytest.mark.parametrize("off", list(SWITCHES))
+@pytest.mark.parametrize("name", ["100.txt", "hepPH_SCS"])
+def test_rule_switches_expand_the_oracles_nodes(graphs, name, off):
+    G, og = graphs[name]
+    r1, r2, r3, p1, p2 = SWITCHES[off]
+    flags = r1 | r2 << 1 | r3 << 2 | p1 << 3 | p2 << 4 | 0x20
+    qs = [tuple(r["query"]) for r in GOLDEN[name] if r["nodes"] <= 5000]
+    cap = 5000
+    print(name, "switch off:", off, flush=True)
+    res, _ = run_engine(G, qs, deterministic=1, node_budget=cap, limit_queries=16, levels=0,
+                        pair=0, red1=r1, red2=r2, red3=r3, prun1=p1, prun2=p2)
+    for r, q in zip(res, qs):
+        o = og.solve(*q, flags=flags, node_cap=cap)
+        assert (r.kl, r.status, r.nodes) == (o["kl"], o["status"], o["nodes"]), (name, q, off)
 
 
 def golden_cases():

@@ -15,7 +15,8 @@ import networkx as nx
 import numpy as np
 import pytest
 
-from oracle.oracle import FLAG_RECOMPUTE, FLAGS_ALL, OracleGraph, ubd_host, ubd_node, unr_bound
+from oracle.oracle import (FLAG_LEVELS, FLAG_PAIR, FLAG_RECOMPUTE, FLAGS_ALL, OracleGraph,
+                           ubd_host, ubd_node, unr_bound)
 from tests.helpers import brute_force, check_witness, csr_from_edges, random_graph
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -108,7 +109,8 @@ def test_bruteforce_random_graphs(seed):
         h = int(rng.integers(l, n + 2))
         bf = brute_force(off, adj, l, h, q)
         for heu in (0, 1):
-            for flags in (FLAGS_ALL, 0, FLAGS_ALL & ~0x20):
+            for flags in (FLAGS_ALL, 0, FLAGS_ALL & ~0x20, FLAGS_ALL | FLAG_LEVELS,
+                          FLAG_LEVELS, FLAGS_ALL | FLAG_LEVELS | FLAG_PAIR, FLAG_PAIR):
                 r = og.solve(l, h, q, heu, int(rng.integers(1, 5)), flags)
                 exp = 0 if bf is None else bf
                 assert r["kl"] == exp, (seed, trial, n, l, h, q, heu, flags, r, bf)
@@ -120,12 +122,60 @@ def test_bruteforce_random_graphs(seed):
         assert plain == (0 if bf is None else bf)
 
 
-def test_recompute_and_trail_solvers_expand_the_same_nodes():
+@pytest.mark.parametrize("seed", range(4))
+def test_pair_rule_on_dense_small_windows(seed):
+    """The pair rule only acts when 2(kl+1) > h - 2: dense graphs with a
+    narrow size window [l, h] put most nodes in that regime.  Every optimum
+    and witness agrees with brute force."""
+    rng = np.random.default_rng(100 + seed)
+    active = 0
+    for trial in range(30):
+        n = int(rng.integers(7, 12))
+        off, adj = random_graph(rng, n, float(rng.uniform(0.5, 0.95)))
+        og = OracleGraph.from_csr(n, off, adj)
+        q = int(rng.integers(0, n))
+        h = int(rng.integers(3, min(n, 8) + 1))
+        l = int(rng.integers(1, h + 1))
+        bf = brute_force(off, adj, l, h, q)
+        exp = 0 if bf is None else bf
+        active += 2 * (exp + 1) > h - 2
+        for flags in (FLAGS_ALL | FLAG_PAIR, FLAGS_ALL | FLAG_LEVELS | FLAG_PAIR, FLAG_PAIR):
+            for heu in (0, 1):
+                r = og.solve(l, h, q, heu, 4, flags)
+                assert r["kl"] == exp, (seed, trial, n, l, h, q, flags, heu, r, bf)
+                if bf is not None and (bf > 0 or l <= 1):
+                    check_witness(off, adj, q, l, h, r["kl"], r["witness"])
+    assert active >= 10
+
+
+@pytest.mark.parametrize("extra", [0, FLAG_LEVELS])
+def test_recompute_and_trail_solvers_expand_the_same_nodes(extra):
     og = OracleGraph.load(os.path.join(GOLD, "hepPH_SCS"))
-    for q in [(3, 6, 2975, 1, 1), (6, 9, 8145, 1, 1), (9, 12, 6792, 1, 1), (12, 15, 5721, 1, 1)]:
-        a = og.solve(*q, flags=FLAGS_ALL, node_cap=4000)
-        b = og.solve(*q, flags=FLAGS_ALL | FLAG_RECOMPUTE, node_cap=4000)
+    for q in [(3, 6, 2975, 1, 1), (6, 9, 8145, 1, 1), (9, 12, 6792, 1, 1), (12, 15, 5721, 1, 1),
+              (9, 12, 6792, 0, 4)]:
+        a = og.solve(*q, flags=FLAGS_ALL | extra, node_cap=4000)
+        b = og.solve(*q, flags=FLAGS_ALL | FLAG_RECOMPUTE | extra, node_cap=4000)
         assert (a["kl"], a["nodes"], a["status"]) == (b["kl"], b["nodes"], b["status"])
+        assert (a["levels"] > 1) == (extra != 0 and a["ku"] > a["heu_kl"] + 1 and a["levels"] > 1)
+
+
+def test_levels_keep_every_golden_optimum():
+    """Descending level floors change the search, never the optimum: every
+    golden query of the shipped graphs gets the same kl with them."""
+    gold = json.load(open(os.path.join(GOLD, "golden.json")))
+    n = 0
+    for name in ("graph.txt", "100.txt", "hepPH_SCS"):
+        og = OracleGraph.load(os.path.join(GOLD, name))
+        off, adj = og.csr()
+        for r in gold[name]:
+            for extra in (FLAG_LEVELS, FLAG_LEVELS | FLAG_PAIR, FLAG_PAIR):
+                o = og.solve(*r["query"], flags=FLAGS_ALL | extra, node_cap=2_000_000)
+                assert (o["kl"], o["status"]) == (r["kl"], r["status"]), (name, r["query"], extra)
+                q = r["query"]
+                if o["kl"] > 0:
+                    check_witness(off, adj, q[2], q[0], q[1], o["kl"], o["witness"])
+            n += 1
+    assert n >= 20
 
 
 def test_monotone_in_h_and_l():
