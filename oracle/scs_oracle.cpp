@@ -491,6 +491,7 @@ struct Params {
   u32 red1, red2, red3, prun1, prun2;
   u32 use_doms;
   u32 pair = 0;
+  u32 root_filter = 0;  // level roots start with R = vertices that can matter at kl
   u64 node_cap;
   u32 dmax;
   u32 ku;
@@ -498,6 +499,15 @@ struct Params {
 
 enum { ST_R = 0, ST_C = 1, ST_X = 2 };
 enum { NODE_PRUNE = 0, NODE_BRANCH = 1 };
+
+// Level root filter: with levels on, a level's root starts with R = the
+// kept vertices other than q with core > kl and dist <= ubd_node(kl, h);
+// the rest start removed.  None of them can lie in an H with min degree >
+// kl and |H| <= h (core(v) <= kl, or farther from q than such an H's
+// diameter), and the root's rule-1/2 closure would remove every one of them.
+static bool root_keeps(const Local& L, const Params& P, u32 kl, u32 v) {
+  return L.core[v] > kl && L.dist[v] <= ubd_node(kl, P.h);
+}
 
 // (a) recompute solver: every node rebuilds dC/dR from the explicit (C, R)
 // lists, as the reference's kernels read them from the task arrays.
@@ -715,7 +725,7 @@ struct RecomputeSolver {
     Task root;
     root.C.push_back(qloc);
     for (u32 v = 0; v < L.K; v++)
-      if (v != qloc) root.R.push_back(v);
+      if (v != qloc && (!P.root_filter || root_keeps(L, P, kl, v))) root.R.push_back(v);
     stack.push_back(std::move(root));
     std::vector<Task> kids;
     while (!stack.empty()) {
@@ -1054,9 +1064,17 @@ struct TrailSolver {
     dC.assign(K, 0);
     dR.assign(K, 0);
     hist.assign(P.h + 3, 0);
-    for (u32 v = 0; v < K; v++) dR[v] = L.off[v + 1] - L.off[v];
-    hist[0] = K;
-    nR = K;
+    if (P.root_filter)
+      for (u32 v = 0; v < K; v++)
+        if (v != qloc && !root_keeps(L, P, kl, v)) st[v] = ST_X;
+    nR = 0;
+    for (u32 v = 0; v < K; v++) {
+      u32 d = 0;
+      for (u32 j = L.off[v]; j < L.off[v + 1]; j++) d += st[L.adj[j]] != ST_X;
+      dR[v] = d;
+      nR += st[v] == ST_R;
+    }
+    hist[0] = nR;
     moveC(qloc);
     trail.clear();
     u32 from = 0, kl_seen = UINT32_MAX;
@@ -1322,6 +1340,7 @@ int orc_solve(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 fl
   P.prun2 = (flags >> 4) & 1;
   P.use_doms = (flags >> 5) & 1;
   P.pair = (flags >> 8) & 1;
+  P.root_filter = (flags >> 7) & 1;
   P.node_cap = node_cap;
   P.dmax = g.dmax;
   P.ku = ku;

@@ -97,6 +97,7 @@ __device__ u32 ubd_node(u32 kl, u32 h) {
 // LDS of one wave (one workgroup).
 struct Lds {
   u32 cnt, cnt2, nfront, ncand, nh, nk, pad0, pad1;
+  u32 root[8];              // payload of a level root this wave starts itself
   u64 tph[SCS_PHASES];      // wall-clock ticks per phase (lane 0 adds)
   u32 D[SCS_FRAME_DOMS];
   u32 hist[SCS_HCAP + 4];   // dC histogram of R (live copy of the wave's state)
@@ -1531,12 +1532,21 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
   const u32* C = task + 4;
   const u32* Lst = task + 4 + nC;
   const u32 K = w.K;
-  const u8 fill = form ? ST_X : ST_R;
   // queued[], nmark[] and acc[] are all-zero between nodes (every node
   // clears what it set), so only the status needs initialising
-  for (u32 v = LANE; v < K; v += 64) w.vs[v].st = fill;
-  wsync();
-  for (u32 i = LANE; i < nL; i += 64) w.vs[Lst[i]].st = form ? ST_R : ST_X;
+  if (form == 2) {
+    // a level's root (C = {q}): R = the vertices that can lie in an H with
+    // min degree > kl and |H| <= h (core > kl, within ubD(kl, h) of q); the
+    // root's rule-1/2 closure would remove every other one
+    const u32 kl = cur_kl(w), ub = ubd_node(kl, w.h);
+    for (u32 v = LANE; v < K; v += 64)
+      w.vs[v].st = (w.core[v] > kl && w.dist[v] <= ub) ? ST_R : ST_X;
+  } else {
+    const u8 fill = form ? ST_X : ST_R;
+    for (u32 v = LANE; v < K; v += 64) w.vs[v].st = fill;
+    wsync();
+    for (u32 i = LANE; i < nL; i += 64) w.vs[Lst[i]].st = form ? ST_R : ST_X;
+  }
   wsync();
   for (u32 i = LANE; i < nC; i += 64) {
     w.vs[C[i]].st = ST_C;
@@ -1702,47 +1712,41 @@ __device__ __forceinline__ void save_state(W& w, u32 active, u32 qslot, u32 from
 
 // The wave that ends a query's last task either starts the query's next
 // level or marks the query done.  A level that ends with no witness above
-// its floor proves opt <= floor: the stopping bound ku drops to the floor
-// and the root task (C = {q}) goes back to the pool with the floor one
-// lower.  Every task of the query has ended and flushed its node count, so
-// the slot's fields are this wave's alone here.
-__device__ void finish_task(W& w, const EngineArgs& A, u32 qslot) {
+// its floor proves opt <= floor: the stopping bound ku drops to the floor,
+// the floor goes one lower, and this wave itself searches the new level's
+// root (return value true; its payload is left in the wave's LDS, and the
+// pool is not involved, so a full ring cannot stop a level).  Every task of the query has ended and flushed its node
+// count, so the slot's fields are this wave's alone here.
+__device__ bool finish_task(W& w, const EngineArgs& A, u32 qslot) {
+  u32 again = 0;
   if (LANE == 0) {
     QuerySlot* Q = &A.slots[qslot];
     u32 left = atomicSub(&Q->pending, 1u) - 1;
     if (left == 0) {
       __threadfence();
       const u32 kl = aload(&Q->kl), fl = aload(&Q->floor);
-      bool again = false;
       if ((Q->flags & QF_LEVELS) && !aload(&Q->stop) && kl < fl) {
-        u64 pos = 0;
-        if (pool_try_enqueue(A.pool, &pos, A.pool.nslots)) {
-          Q->ku = min(aload(&Q->ku), fl);
-          Q->floor = fl - 1;
-          Q->lvl_nodes0 = aload64(&Q->nodes);
-          Q->pending = 1;
-          u32* dst = A.pool.data + (pos % A.pool.nslots) * (u64)A.pool.slot_words;
-          dst[0] = qslot;
-          dst[1] = 1;
-          dst[2] = 0;
-          dst[3] = 0;
-          dst[4] = Q->qloc;
-          __threadfence();
-          astore64(&A.pool.seq[pos % A.pool.nslots], pos + 1);
-          again = true;
-        } else {
-          // no pool slot (root slots are reserved, so not expected): the
-          // result is reported as a lower bound
-          atomicExch(&Q->capped, 1u);
-        }
-      }
-      if (!again) {
+        Q->ku = min(aload(&Q->ku), fl);
+        Q->floor = fl - 1;
+        Q->lvl_nodes0 = aload64(&Q->nodes);
+        Q->pending = 1;
+        u32* rt = w.L->root;
+        rt[0] = qslot;
+        rt[1] = 1;
+        rt[2] = 0;
+        rt[3] = 2;  // level root
+        rt[4] = Q->qloc;
+        __threadfence();
+        again = 1;
+      } else {
         atomicExch(&Q->state, (u32)QS_DONE);
         atomicSub(&A.glob->active_q, 1u);
       }
     }
   }
+  again = bcast(again);
   wsync();
+  return again != 0;
 }
 
 __device__ void flush_counters(W& w, const EngineArgs& A, u32& pend) {
@@ -1932,7 +1936,15 @@ __global__ void __launch_bounds__(64, 2) __attribute__((flatten)) scs_epoch(Engi
       if (!cont) {
         active = 0;
         flush_counters(w, A, pend);
-        finish_task(w, A, qslot);
+        if (finish_task(w, A, qslot)) {
+          // the query's next level: its root, searched by this wave
+          ph_switch(&L, tp, ph, PH_REBUILD);
+          rebuild(w, L.root);
+          active = 1;
+          from = 0;
+          kl_seen = INV;
+          wsync();
+        }
         continue;
       }
     }
@@ -2038,7 +2050,7 @@ __global__ void __launch_bounds__(64) scs_admit(EngineArgs A, const u32* slots_i
       dst[0] = sl;
       dst[1] = 1;
       dst[2] = 0;
-      dst[3] = 0;
+      dst[3] = (A.slots[sl].flags & QF_LEVELS) ? 2u : 0u;  // level root or plain root
       dst[4] = A.slots[sl].qloc;
       failed[i] = 0;
       __threadfence();

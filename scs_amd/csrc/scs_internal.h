@@ -118,7 +118,8 @@ struct WaveLayout {
 };
 
 // Pool of donated tasks: bounded MPMC ring (per-slot sequence numbers).
-// Task payload: [qslot, nC, nL, form(0 = list is X, 1 = list is R)], C[nC], L[nL].
+// Task payload: [qslot, nC, nL, form(0 = list is X, 1 = list is R, 2 = a level root:
+// C = {q}, R = the kept vertices with core > kl within ubD(kl, h))], C[nC], L[nL].
 struct Pool {
   u64* seq;        // nslots
   u32* data;       // nslots * slot_words

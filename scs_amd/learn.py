@@ -4,15 +4,20 @@
    every stride-th search node records up to 64 of its branching
    candidates with six features (dC, dR, dist(q, v), core(v),
    slack = dC + min(dR, h-|C|-1) - kl, and the plain connection score).
-2. Labels.  A candidate is positive when it belongs to the best community
-   H the search found for its query (the witness), i.e. choosing it leads
-   towards the optimum.
+2. Labels.  Only queries whose optimum the search proved give rows: a
+   candidate is positive when it belongs to the optimal community H the
+   search returned for its query (the witness), i.e. choosing it leads to
+   the optimum.  Rows of budget-capped queries are dropped, since their
+   witness marks only a lower bound.
 3. Training.  The 6-64-64-1 MLP is trained on the GPU by the library's own
    forward/backward/Adam kernels (scs_mlp_train) with class-balanced binary
    cross-entropy.
-4. Evaluation.  The same query stream is run with the plain connection-score
-   rule and with the learned order; both return the same optimum (the order
-   never prunes), and the node counts and times are compared.
+4. Evaluation.  Unseen queries (the workload's own stream) are run with the
+   plain connection-score rule and with the learned order at the same node
+   budget; every optimum both prove is the same (the order never prunes),
+   and on the queries both prove the nodes expanded and the time to the
+   optimum (submission to result) are compared, beside how many each
+   proves.
 
 Usage:
   python -m scs_amd.learn --config C1 --out models/branching_c1.npz
@@ -45,13 +50,18 @@ def training_queries(name: str, core: np.ndarray, seeds=range(100, 200), per_see
     return [(q.l, q.h, q.q, q.heu, q.limit_doms) for q in out]
 
 
+PROVEN = ("OK", "HEU_OPT")
+
+
 def collect(G, queries, budget, stride=2, cap=2_000_000, **engine_kw):
     E = scs.Engine(G, node_budget=budget, **engine_kw)
     E.trace(cap, stride)
     res = E.run(queries, tags=list(range(len(queries))), progress=10.0)
     rows = E.trace_read()
     E.close()
-    wit = {r.tag: set(int(x) for x in r.witness) for r in res}
+    wit = {r.tag: set(int(x) for x in r.witness) for r in res
+           if r.status in PROVEN and r.kl > 0}
+    rows = rows[np.isin(rows[:, 0], np.array(sorted(wit), dtype=rows.dtype))] if wit else rows[:0]
     X = np.zeros((len(rows), 6), np.float32)
     X[:, 0] = rows[:, 2]
     X[:, 1] = rows[:, 3]
@@ -89,7 +99,18 @@ def evaluate(G, queries, model, budget, **engine_kw):
         proven = sum(r.status in ("OK", "HEU_OPT", "INFEASIBLE") for r in res)
         out[mode] = {"seconds": dt, "nodes": int(st["nodes"]), "proven": proven,
                      "kl": [r.kl for r in res], "status": [r.status for r in res],
-                     "nodes_per_query": [int(r.nodes) for r in res]}
+                     "nodes_per_query": [int(r.nodes) for r in res],
+                     "latency_ms": [r.latency_us / 1000.0 for r in res]}
+    p, l = out["plain"], out["learned"]
+    both = [i for i, (a, b) in enumerate(zip(p["status"], l["status"]))
+            if a != "CAPPED" and b != "CAPPED"]
+    out["both_proven"] = {
+        "queries": len(both),
+        "optimum_equal": all(p["kl"][i] == l["kl"][i] for i in both),
+        "nodes_plain": int(sum(p["nodes_per_query"][i] for i in both)),
+        "nodes_learned": int(sum(l["nodes_per_query"][i] for i in both)),
+        "time_to_optimum_ms_plain": round(sum(p["latency_ms"][i] for i in both), 1),
+        "time_to_optimum_ms_learned": round(sum(l["latency_ms"][i] for i in both), 1)}
     return out
 
 
@@ -104,7 +125,10 @@ def main():
     ap.add_argument("--out", default=os.path.join(ROOT, "models", "branching_c1.npz"))
     ap.add_argument("--report", default=None)
     ap.add_argument("--limit-queries", type=int, default=1024)
+    ap.add_argument("--levels", type=int, default=1)
+    ap.add_argument("--pair", type=int, default=1)
     args = ap.parse_args()
+    ekw = dict(limit_queries=args.limit_queries, levels=args.levels, pair=args.pair)
 
     print(f"[learn] building {args.config}", file=sys.stderr, flush=True)
     g = configs.graph(args.config)
@@ -112,10 +136,11 @@ def main():
     core = G.core().astype(np.int64)
     tq = training_queries(args.config, core, per_seed=args.per_seed)
     t0 = time.perf_counter()
-    X, y, _ = collect(G, tq, args.budget, stride=args.stride, limit_queries=args.limit_queries)
+    X, y, res = collect(G, tq, args.budget, stride=args.stride, **ekw)
+    n_proven = sum(r.status in PROVEN for r in res)
     t_col = time.perf_counter() - t0
-    print(f"[learn] {len(tq)} training queries -> {len(y)} rows, {int(y.sum())} positive "
-          f"({t_col:.1f} s)", file=sys.stderr, flush=True)
+    print(f"[learn] {len(tq)} training queries ({n_proven} proven) -> {len(y)} rows, "
+          f"{int(y.sum())} positive ({t_col:.1f} s)", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     m, loss, pw = train(X, y, epochs=args.epochs)
     t_tr = time.perf_counter() - t0
@@ -125,16 +150,18 @@ def main():
     m.save(args.out)
     eq = [(q.l, q.h, q.q, q.heu, q.limit_doms)
           for q in configs.queries(args.config, core, count=args.eval_queries)]
-    ev = evaluate(G, eq, m, args.budget, limit_queries=args.limit_queries)
+    ev = evaluate(G, eq, m, args.budget, **ekw)
     same = all(a == b for a, b, s1, s2 in zip(ev["plain"]["kl"], ev["learned"]["kl"],
                                                  ev["plain"]["status"], ev["learned"]["status"])
                if s1 != "CAPPED" and s2 != "CAPPED")
-    rep = {"config": args.config, "train_queries": len(tq), "rows": int(len(y)),
+    rep = {"config": args.config, "train_queries": len(tq), "train_proven": n_proven,
+           "levels": args.levels, "pair": args.pair, "rows": int(len(y)),
            "positives": int(y.sum()), "loss": loss, "collect_s": t_col, "train_s": t_tr,
            "eval_queries": len(eq), "budget": args.budget,
            "plain": {k: ev["plain"][k] for k in ("seconds", "nodes", "proven")},
            "learned": {k: ev["learned"][k] for k in ("seconds", "nodes", "proven")},
            "optimum_equal_where_both_complete": same,
+           "both_proven": ev["both_proven"],
            "kl_sum_plain": int(sum(ev["plain"]["kl"])), "kl_sum_learned": int(sum(ev["learned"]["kl"]))}
     print(json.dumps(rep), flush=True)
     if args.report:

@@ -5,7 +5,7 @@
 //              <limitQueries> <factor> <red1> <red2> <red3> <prun1> <prun2>
 //              [--wave-per-cu N] [--limit-tasks N] [--node-budget N]
 //              [--mem-budget BYTES] [--learned MODEL.bin] [--time-slice-us N]
-//              [--results DIR] [--queue-key C]
+//              [--results DIR] [--queue-key C] [--levels 0|1] [--pair 0|1]
 //
 // The twelve positional arguments are those of the reference (main.cu:
 // 481-502).  A listener thread receives "<l> <h> <q> <heu> <limitDoms>"
@@ -98,6 +98,8 @@ int main(int argc, char** argv) {
     else if (a == "--node-budget") p.node_budget = strtoull(val(), nullptr, 10);
     else if (a == "--mem-budget") p.mem_budget = strtoull(val(), nullptr, 10);
     else if (a == "--time-slice-us") p.time_slice_us = (uint32_t)atoi(val());
+    else if (a == "--levels") p.levels = (uint32_t)atoi(val());
+    else if (a == "--pair") p.pair = (uint32_t)atoi(val());
     else if (a == "--learned") model_path = val();
     else if (a == "--results") results_dir = val();
     else if (a == "--queue-key") key = val()[0];
