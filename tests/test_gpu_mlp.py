@@ -89,22 +89,20 @@ def test_learned_order_keeps_every_optimum():
     d["mean"][:6] = [3, 30, 2, 20, 10, 10]
     d["scale"][:6] = [0.3, 0.03, 0.5, 0.05, 0.1, 0.1]
     m = scs.Model.from_arrays(d)
-    rows = gold["C1"] + gold["hepPH_SCS"][:0]
+    rows = [r for r in gold["C1"] if r["status"] != "CAPPED" and r["nodes"] <= 20000]
     qs = [tuple(r["query"]) for r in rows]
-    # an untrained order can make the tree far larger: bound it; a complete
-    # search must return the optimum, a capped one a lower bound of it
-    E = scs.Engine(G, limit_queries=16, node_budget=400_000)
+    # every golden C1 query the oracle proved within 20,000 nodes must also
+    # complete in the (untrained, arbitrary) learned order, with the same
+    # optimum: the order changes which vertex is branched on, never what is
+    # pruned.  The budget only bounds a regression.
+    E = scs.Engine(G, limit_queries=64, node_budget=5_000_000)
     E.set_model(m, True)
     res = E.run(qs)
     E.close()
-    complete = 0
+    assert len(rows) >= 20
     for r, row in zip(res, rows):
-        if r.status == "CAPPED":
-            assert r.heu_kl <= r.kl <= row["kl"]
-        else:
-            complete += 1
-            assert r.kl == row["kl"] and r.status in ("OK", "HEU_OPT")
-    assert complete >= 1
+        assert r.status in ("OK", "HEU_OPT"), (row["query"], r.status)
+        assert r.kl == row["kl"], (row["query"], r.kl, row["kl"])
     G.close()
 
 

@@ -259,3 +259,29 @@ def test_golden_vectors_reproduce():
             q = r["query"]
             if r["kl"] > 0:
                 check_witness(off, adj, q[2], q[0], q[1], r["kl"], r["witness"])
+
+
+def test_c1_golden_reproduces_and_witnesses_hold():
+    """The C1 golden file: every proven entry's witness is a valid community
+    of its kl, every entry's kl is within [heu_kl, ku], and the entries the
+    oracle proved in at most 5,000 nodes reproduce node for node with the
+    recorded flags and cap."""
+    from scs_amd import configs
+    gold = json.load(open(os.path.join(GOLD, "golden.json")))
+    g = configs.graph("C1")
+    assert g.sha256() == gold["C1_sha256"]
+    og = OracleGraph.from_csr(g.n, g.off, g.adj)
+    srch = gold["C1_search"]
+    rows = gold["C1"]
+    assert len(rows) == 100 and srch["proven"] == sum(r["status"] != "CAPPED" for r in rows)
+    redo = 0
+    for r in rows:
+        q = r["query"]
+        assert r["heu_kl"] <= r["kl"] <= r["ku"]
+        if r["kl"] > 0:
+            check_witness(g.off, g.adj, q[2], q[0], q[1], r["kl"], r["witness"])
+        if r["status"] != "CAPPED" and r["nodes"] <= 5000:
+            o = og.solve(*q, flags=srch["flags"], node_cap=srch["node_cap"])
+            assert (o["kl"], o["status"], o["nodes"]) == (r["kl"], r["status"], r["nodes"]), q
+            redo += 1
+    assert redo >= 20
