@@ -140,7 +140,19 @@ int scs_graph_core(const scs_graph* g, uint32_t* core);
 int scs_graph_csr(const scs_graph* g, int64_t* off, int32_t* adj);
 /* Device time of the core decomposition in microseconds. */
 double scs_graph_core_time_us(const scs_graph* g);
-
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the core numbers scs_graph_create computes, for a CSR
+ * already in device memory (d_off: n+1 int64, d_adj: int32, both device
+ * pointers, a simple symmetric graph), written to d_core (n words, device),
+ * on `stream` (a hipStream_t; NULL = the default stream).  Working memory is
+ * a fixed device array inside the library (3 * 2^20 + 64 words, 12 MB per
+ * GPU, held until the process ends), so n is at most 2^20 (SCS_EBIG beyond).
+ * Allocates nothing.  Every call on one GPU uses that GPU's array: calls
+ * that overlap on one GPU race. */
+#define SCS_CORE_NOALLOC_MAX_N (1u << 20)
+int scs_core_numbers_noalloc(const int64_t* d_off, const int32_t* d_adj, uint32_t n,
+                             uint32_t* d_core, void* stream);
 void scs_graph_destroy(scs_graph* g);
 
 void scs_params_default(scs_params* p);

@@ -4,6 +4,8 @@
 // scs_prep.hip.
 #include "scs_internal.h"
 
+#include <mutex>
+
 #define BLK 256
 
 // ---------------------------------------------------------------------------
@@ -88,4 +90,94 @@ extern "C" hipError_t scs_launch_core_peel(const i64* off, const int32_t* adj, u
   core_peel<<<nblocks((u64)fn * 64, BLK, 8192), BLK, 0, s>>>(off, adj, k, F, fn, deg, core, Fn,
                                                              fncount);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Core numbers without allocation, for callers that cannot allocate device
+// memory.  The working memory (degrees, two frontiers, three counters) is
+// this fixed device array; the host finds its per-device address once with
+// hipGetSymbolAddress and passes that address to the kernels.
+// Layout (words): deg[CORE_NA_MAX_N] | F[..] | Fn[..] | counters[64].
+// ---------------------------------------------------------------------------
+#define CORE_NA_MAX_N (1u << 20)
+static __device__ __align__(256) u32 scratch[3u * CORE_NA_MAX_N + 64];
+
+#define CORE_NA_DEVICES 64
+static u32* g_scratch_dev[CORE_NA_DEVICES];
+static std::mutex g_scratch_mu;
+
+// cnt[0..3) = a, b, c (a kernel, not a memset: callers may capture this)
+__global__ void core_counters(u32* cnt, u32 a, u32 b, u32 c) {
+  if (threadIdx.x == 0) {
+    cnt[0] = a;
+    cnt[1] = b;
+    cnt[2] = c;
+  }
+}
+
+static int scratch_address(u32** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= CORE_NA_DEVICES) return SCS_EHIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SCS_EHIP;
+    g_scratch_dev[dev] = (u32*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return SCS_OK;
+}
+
+#define NA_CHK(x)                          \
+  do {                                     \
+    if ((x) != hipSuccess) return SCS_EHIP; \
+  } while (0)
+
+extern "C" int scs_core_numbers_noalloc(const int64_t* d_off, const int32_t* d_adj, uint32_t n,
+                                        uint32_t* d_core, void* stream) {
+  if (!d_off || !d_core || (n && !d_adj)) return SCS_EINVAL;
+  if (n > CORE_NA_MAX_N) return SCS_EBIG;
+  if (n == 0) return SCS_OK;
+  u32* S = nullptr;
+  const int rc = scratch_address(&S);
+  if (rc) return rc;
+  u32* deg = S;
+  u32* F = S + CORE_NA_MAX_N;
+  u32* Fn = S + 2u * CORE_NA_MAX_N;
+  u32* cnt = S + 3u * CORE_NA_MAX_N;
+  hipStream_t s = (hipStream_t)stream;
+  NA_CHK(scs_launch_core_init(d_off, n, deg, d_core, s));
+  u32 peeled = 0, k = 0;
+  while (peeled < n) {
+    // cnt[0] = frontier size, cnt[1] = min remaining degree
+    core_counters<<<1, 64, 0, s>>>(cnt, 0u, 0xffffffffu, 0u);
+    NA_CHK(hipGetLastError());
+    NA_CHK(scs_launch_core_scan(n, k, deg, d_core, F, cnt, cnt + 1, s));
+    u32 h[2];
+    NA_CHK(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+    NA_CHK(hipStreamSynchronize(s));
+    u32 fn = h[0];
+    if (fn == 0) {
+      if (h[1] == 0xffffffffu) break;
+      k = h[1] > k + 1 ? h[1] : k + 1;
+      continue;
+    }
+    peeled += fn;
+    u32 *cur = F, *nxt = Fn;
+    while (fn) {
+      core_counters<<<1, 64, 0, s>>>(cnt, 0u, 0xffffffffu, 0u);
+      NA_CHK(hipGetLastError());
+      NA_CHK(scs_launch_core_peel(d_off, d_adj, k, cur, fn, deg, d_core, nxt, cnt + 2, s));
+      u32 f2 = 0;
+      NA_CHK(hipMemcpyAsync(&f2, cnt + 2, sizeof(u32), hipMemcpyDeviceToHost, s));
+      NA_CHK(hipStreamSynchronize(s));
+      fn = f2;
+      peeled += fn;
+      u32* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    k++;
+  }
+  return SCS_OK;
 }
