@@ -1,17 +1,26 @@
-"""Benchmark: size-bounded community search queries per second on one MI355X.
+"""Benchmark: proven-optimal size-bounded community search queries per second
+on one MI355X.
 
-Workload (default): the batched GPU configuration, a Chung-Lu graph (seed 2,
-n = 100,000, 1.5M edges, exponent 2.3) with 2,000 queries (l in [5,40],
-h = l + [5,60], q over core >= 8, heuristic on, limit_dominating_set 8),
-all submitted at once to the persistent engine.  A step is one pass of the
-whole query batch through the engine (admission: heuristic + GPU
-preprocessing; search; results).  Each query runs until its optimum is
-proven or its node budget is spent (status CAPPED, kl is then a lower
-bound); the fraction proven optimal is reported beside the rate.
+Workload (default): the serving configuration, a Chung-Lu graph (seed 4,
+n = 500,000, 8M edges, exponent 2.2) and its query stream (l in [5,50],
+h = l + [5,150], q over every vertex, heuristic 0/1 at random,
+limit_dominating_set in {4,8,16}).  A step is one batch of consecutive
+queries of the stream (64 by default) submitted together to the persistent
+engine (admission: heuristic + batched GPU preprocessing; search; results),
+each query searched until its optimum is proven or its node budget is spent
+(status CAPPED: kl is then only a lower bound).  Warm-up steps take the
+stream's first batches and the timed steps the batches after them, so no
+query is timed twice.
 
-Prints one JSON line (rank 0).  The CPU baseline is the project's serial
-exact solver (the oracle) on a bounded sample of the same queries with the
-same node budget, timed on this host.
+value = proven-optimal queries per second over the timed steps: CAPPED
+answers are excluded from the numerator and counted beside it, with the
+proven fraction.  Beside it the line reports, on the same timed queries:
+the learned branching order (batched), the plain order one query at a time,
+the CPU oracle (same rules, same budget, one core, on a bounded sample) and
+whether every optimum both proved is equal; the preprocessing kernels alone;
+and the roofline of the search kernel.
+
+Prints one JSON line (rank 0).
 """
 
 from __future__ import annotations
@@ -19,6 +28,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -32,8 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # neighbour id plus the neighbour's state read and written (status 1 B,
 # dC 2 B, dR 4 B read; dC 2 B and dR 4 B written)
 BYTES_PER_EDGE = 4 + 7 + 6
-
-
+PROVEN = ("OK", "HEU_OPT", "INFEASIBLE")
 PHASES = ("rebuild", "rules", "bounds", "score", "doms", "move", "donate", "idle", "closure_scan",
           "closure_remove", "argmax", "clear", "pair")
 
@@ -49,6 +58,39 @@ def percentile(xs, p):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), p)) if len(xs) else 0.0
 
 
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def summary(res, wall):
+    proven = [r for r in res if r.status in PROVEN]
+    lat = [r.latency_us / 1000.0 for r in proven]
+    st = {}
+    for r in res:
+        st[r.status] = st.get(r.status, 0) + 1
+    return {"value": round(len(proven) / wall, 3) if wall > 0 else 0.0, "unit": "queries/s",
+            "proven": len(proven), "queries": len(res),
+            "proven_optimal_fraction": round(len(proven) / max(1, len(res)), 4),
+            "p50_ms": round(percentile(lat, 50), 3), "p99_ms": round(percentile(lat, 99), 3),
+            "status_counts": st, "nodes": int(sum(r.nodes for r in res)), "wall_s": round(wall, 3)}
+
+
+def run_batches(E, batches, progress=None):
+    """Each batch submitted at once and drained before the next; tags number
+    the queries across batches."""
+    out = []
+    for b in batches:
+        base = len(out)
+        out.extend(E.run(b, tags=list(range(base, base + len(b))), progress=progress))
+    return out
+
+
 def prep_leg(scs, G, core, queries):
     """Batched preprocessing of the queries alone (64 per batch, local graphs
     built on the device, then released), with no search kernel on the GPU:
@@ -61,7 +103,8 @@ def prep_leg(scs, G, core, queries):
         kl0, ubd = EP.heuristic(l, h, q)
         if kl0 < min(int(core64[q]), h - 1):
             todo.append((q, kl0, ubd))
-    EP.prep_many(*zip(*todo[:1])) if todo else None  # code objects
+    if todo:
+        EP.prep_many(*zip(*todo[:1]))  # code objects
     p0 = EP.stats()
     Ks = []
     for c0 in range(0, len(todo), 64):
@@ -80,34 +123,88 @@ def prep_leg(scs, G, core, queries):
             "mean_kept": round(float(np.mean(Ks)), 1) if Ks else 0}
 
 
+def extra_leg(scs, configs, name, args, eng_kw):
+    """A further workload beside the headline.  C1: all 100 queries of its
+    stream in one batch at the golden file's node cap and rules, every
+    optimum the GPU proves compared with the golden optimum where the file
+    has one.  Other workloads: batches of the stream like the headline's."""
+    print(f"[bench] extra workload {name}", file=sys.stderr, flush=True)
+    g = configs.graph(name)
+    G = scs.Graph.from_csr(g.off, g.adj)
+    core = G.core().astype(np.int64)
+    out = {}
+    if name == "C1":
+        gold = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+        rows = gold["C1"]
+        qs = [tuple(r["query"]) for r in rows]
+        kw = dict(eng_kw, node_budget=gold["C1_search"]["node_cap"], deadline_ms=0)
+        E = scs.Engine(G, **kw)
+        E.run([qs[0]])
+        t0 = time.perf_counter()
+        res = E.run(qs, tags=list(range(len(qs))), progress=20.0)
+        wall = time.perf_counter() - t0
+        E.close()
+        out = summary(res, wall)
+        both = [(r, row) for r, row in zip(res, rows)
+                if r.status in PROVEN and row["status"] != "CAPPED"]
+        out.update({"node_budget": kw["node_budget"], "golden_proven": gold["C1_search"]["proven"],
+                    "both_proven": len(both),
+                    "equal_to_golden": sum(r.kl == row["kl"] for r, row in both),
+                    "above_golden_lower_bound": sum(
+                        r.kl >= row["kl"] for r, row in zip(res, rows)
+                        if row["status"] == "CAPPED" and r.status in PROVEN)})
+    else:
+        B = args.batch
+        qs = configs.queries(name, core, count=args.extra_batches * B)
+        batches = [[(q.l, q.h, q.q, q.heu, q.limit_doms) for q in qs[i * B:(i + 1) * B]]
+                   for i in range(args.extra_batches)]
+        E = scs.Engine(G, **eng_kw)
+        E.run([batches[0][0]])
+        t0 = time.perf_counter()
+        res = run_batches(E, batches, progress=20.0)
+        wall = time.perf_counter() - t0
+        E.close()
+        out = summary(res, wall)
+        out["batches"] = f"{args.extra_batches} x {B}"
+    G.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--config", default="C2")
-    ap.add_argument("--queries", type=int, default=2000)
-    ap.add_argument("--budget", type=int, default=300, help="node budget per query")
-    # defaults: the autotuner's pick for this workload (scs_amd/tuned_c2.json)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="C4")
+    ap.add_argument("--batch", type=int, default=64, help="queries per step")
+    ap.add_argument("--budget", type=int, default=20000, help="node budget per query")
+    ap.add_argument("--deadline-ms", type=int, default=2000,
+                    help="per-query wall-clock bound from submission (CAPPED past it)")
     ap.add_argument("--heuristic-threads", type=int, default=12)
     ap.add_argument("--waves-per-cu", type=int, default=4)
-    ap.add_argument("--limit-queries", type=int, default=2048)
-    ap.add_argument("--slice-us", type=int, default=200000)
+    ap.add_argument("--limit-queries", type=int, default=256)
+    ap.add_argument("--slice-us", type=int, default=50000)
     ap.add_argument("--copy-limit", type=float, default=0.5)
-    ap.add_argument("--admit-per-step", type=int, default=256,
-                    help="queries admitted (heuristic + GPU preprocessing) per epoch")
+    ap.add_argument("--admit-per-step", type=int, default=256)
+    ap.add_argument("--levels", type=int, default=1, help="descending level floors")
+    ap.add_argument("--pair", type=int, default=1, help="common-neighbour pair rule")
     ap.add_argument("--learned", type=int, default=0,
                     help="1 = the main leg itself uses the learned order")
-    ap.add_argument("--model", default=os.path.join(ROOT, "models", "branching_c1.npz"),
-                    help="branching model for the learned leg (skipped if the file is absent)")
-    ap.add_argument("--prep-sample", type=int, default=512,
+    ap.add_argument("--model", default=None,
+                    help="branching model (default models/branching_<config>.npz when present)")
+    ap.add_argument("--prep-sample", type=int, default=256,
                     help="queries preprocessed alone (batched) for the preprocessing roofline")
-    ap.add_argument("--single-sample", type=int, default=24,
-                    help="queries run one at a time (plain order) for the one-query GPU form")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--single-sample", type=int, default=16,
+                    help="timed queries also run one at a time (plain order)")
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prep-only", action="store_true",
                     help="only run the standalone batched-preprocessing leg (--prep-sample queries)")
+    ap.add_argument("--extra", default="C1,C3",
+                    help="further workloads reported beside the headline: C1 (all 100 queries at "
+                         "the golden file's node cap, optima checked against it) and/or C3 "
+                         "(--extra-batches batches like the headline's)")
+    ap.add_argument("--extra-batches", type=int, default=5)
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibrate-mib", type=int, default=0,
                     help="only run a plain elementwise pass over this many MiB (read once, "
@@ -131,51 +228,52 @@ def main():
               "and would be sharded across GPUs with no collective on the hot path", file=sys.stderr)
         sys.exit(2)
 
-    from scs_amd import configs, graphgen as gg, scs
+    from scs_amd import configs, scs
 
     print(f"[bench] building {args.config}", file=sys.stderr, flush=True)
     g = configs.graph(args.config)
+    cfg = configs.CONFIGS[args.config]
     print(f"[bench] graph n={g.n} m={g.m}", file=sys.stderr, flush=True)
     G = scs.Graph.from_csr(g.off, g.adj)
     core = G.core()
-    qs = configs.queries(args.config, core.astype(np.int64), count=args.queries)
-    queries = [(q.l, q.h, q.q, q.heu, q.limit_doms) for q in qs]
-    model = scs.Model.load(args.model) if args.model and os.path.exists(args.model) else None
+    B = args.batch
+    need = (args.warmup + args.steps) * B
+    qs = configs.queries(args.config, core.astype(np.int64), count=max(need, args.prep_sample))
+    allq = [(q.l, q.h, q.q, q.heu, q.limit_doms) for q in qs]
+    warm = [allq[i * B:(i + 1) * B] for i in range(args.warmup)]
+    timed = [allq[(args.warmup + i) * B:(args.warmup + i + 1) * B] for i in range(args.steps)]
+    timed_flat = [q for b in timed for q in b]
+    mpath = args.model or os.path.join(ROOT, "models", f"branching_{args.config.lower()}.npz")
+    model = scs.Model.load(mpath) if os.path.exists(mpath) else None
     eng_kw = dict(node_budget=args.budget, waves_per_cu=args.waves_per_cu,
                   limit_queries=args.limit_queries, time_slice_us=args.slice_us,
                   copy_limit=args.copy_limit, mem_budget=32 << 30,
-                  heuristic_threads=args.heuristic_threads, admit_per_step=args.admit_per_step)
+                  heuristic_threads=args.heuristic_threads, admit_per_step=args.admit_per_step,
+                  levels=args.levels, pair=args.pair, deadline_ms=args.deadline_ms)
     if args.prep_only:
         print(json.dumps({"workload": args.config, "prep_standalone": prep_leg(
-            scs, G, core, queries[:args.prep_sample or len(queries)])}), flush=True)
+            scs, G, core, allq[:args.prep_sample or len(allq)])}), flush=True)
         return
     E = scs.Engine(G, **eng_kw)
     if args.learned:
         if model is None:
-            sys.exit("--learned needs --model")
+            sys.exit("--learned needs a model")
         E.set_model(model, True)
-    # load code objects outside the timed region
-    E.run([queries[0]])
-
-    for _ in range(args.warmup):
-        E.run(queries)
+    # code objects load outside the timed region
+    E.run([allq[0]])
+    if warm:
+        print(f"[bench] warmup: {args.warmup} batches of {B}", file=sys.stderr, flush=True)
+        run_batches(E, warm)
 
     s0 = E.stats()
+    print(f"[bench] timed: {args.steps} batches of {B}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    all_res = []
-    for _ in range(args.steps):
-        all_res = E.run(queries, progress=10.0)
+    res = run_batches(E, timed, progress=20.0)
     t1 = time.perf_counter()
     s1 = E.stats()
-
+    E.close()
     wall = t1 - t0
-    nq = len(queries) * args.steps
-    qps = nq / wall
-    lat_ms = [r.latency_us / 1000.0 for r in all_res]
-    status = {}
-    for r in all_res:
-        status[r.status] = status.get(r.status, 0) + 1
-    proven = sum(status.get(k, 0) for k in ("OK", "HEU_OPT", "INFEASIBLE")) / max(1, len(all_res))
+    main_sum = summary(res, wall)
     nodes = s1["nodes"] - s0["nodes"]
     # roofline of the dominant kernel (scs_epoch) over every launch of the
     # process, so the average launch duration matches a kernel-trace profile
@@ -184,10 +282,11 @@ def main():
     epochs = s1["epochs"]
     edges = s1["edges"]
     ach = (edges * BYTES_PER_EDGE) / (engine_us * 1e-6) / 1e9 if engine_us > 0 else 0.0
-
+    gdesc = (f"Chung-Lu n={g.n} m={g.m}" if cfg["kind"] == "chung_lu" else
+             f"R-MAT scale {cfg['scale']} n={g.n} m={g.m}")
     line = {
-        "metric": "queries/sec (p99 latency, proven-optimal fraction beside it)",
-        "value": round(qps, 3),
+        "metric": "proven-optimal queries/sec (p99 latency beside it), learned vs plain branching",
+        "value": main_sum["value"],
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -197,110 +296,137 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": "synthetic (deterministic Chung-Lu graph and query stream)",
-        "config": {"workload": f"{args.config}: Chung-Lu n={g.n} m={g.m}, {len(queries)} queries",
-                   "node_budget": args.budget, "learned": bool(args.learned),
-                   "waves": s1["waves"], "limit_queries": args.limit_queries},
-        "p50_ms": round(percentile(lat_ms, 50), 3),
-        "p99_ms": round(percentile(lat_ms, 99), 3),
-        "proven_optimal_fraction": round(proven, 4),
-        "status_counts": status,
+        "data": "synthetic (deterministic generated graph and query stream)",
+        "config": {"workload": f"{args.config}: {gdesc}, {args.steps} batches of {B} queries "
+                               f"from the stream",
+                   "node_budget": args.budget, "deadline_ms": args.deadline_ms,
+                   "learned": bool(args.learned),
+                   "levels": args.levels, "pair": args.pair, "waves": s1["waves"],
+                   "limit_queries": args.limit_queries},
+        "proven": main_sum["proven"],
+        "queries": main_sum["queries"],
+        "proven_optimal_fraction": main_sum["proven_optimal_fraction"],
+        "p50_ms": main_sum["p50_ms"],
+        "p99_ms": main_sum["p99_ms"],
+        "p99_ms_all": round(percentile([r.latency_us / 1000.0 for r in res], 99), 3),
+        "status_counts": main_sum["status_counts"],
         "nodes": int(nodes),
         "nodes_per_s": round(nodes / wall, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 6), "traffic": None,
                      "kernel": "scs_epoch", "epoch_avg_us": round(engine_us / max(1, epochs), 2),
-                     "bytes_per_edge": BYTES_PER_EDGE, "edges": int(edges)},
+                     "epochs": int(epochs), "bytes_per_edge": BYTES_PER_EDGE, "edges": int(edges),
+                     "algorithmic_bytes_per_launch": int(edges * BYTES_PER_EDGE / max(1, epochs))},
         "prep": {"device_us": round(s1["prep_us"] - s0["prep_us"], 1),
                  "algorithmic_bytes": int(s1["prep_bytes"] - s0["prep_bytes"])},
         "search_phases": phase_shares(s0, s1),
-        "domination": {k: int(s1[k] - s0[k]) for k in ("dom_front", "dom_cand", "dom_found")},
-        "rule12_full_rescans": {k: int(s1[k] - s0[k]) for k in ("full_scans", "full_rows")},
         "heuristic_host_us": round(s1["heu_us"] - s0["heu_us"], 1),
         "core_decomposition_us": round(s1["core_us"], 1),
+        "kcap": int(s1["kcap"]),
     }
+    pb, pu = line["prep"]["algorithmic_bytes"], line["prep"]["device_us"]
+    line["prep"]["achieved_GBs"] = round(pb / (pu * 1e-6) / 1e9, 2) if pu > 0 else 0.0
     # HBM traffic per launch from the committed hardware-counter profile of
     # this workload and engine setting (two rocprofv3 --pmc runs, FETCH_SIZE
     # and WRITE_SIZE, scaled by a copy pass of known size)
-    prof = os.path.join(ROOT, "profiles", "scs_epoch_c2_traffic.json")
+    prof = os.path.join(ROOT, "profiles", f"scs_epoch_{args.config.lower()}_traffic.json")
     if os.path.exists(prof):
         t = json.load(open(prof))
-        if (t["workload"] == args.config and t["waves_per_cu"] == args.waves_per_cu and
-                t["time_slice_us"] == args.slice_us and t["node_budget"] == args.budget and
-                not args.learned):
+        same = all(t.get(k) == v for k, v in (("workload", args.config),
+                                              ("waves_per_cu", args.waves_per_cu),
+                                              ("time_slice_us", args.slice_us),
+                                              ("node_budget", args.budget),
+                                              ("batch", B), ("levels", args.levels),
+                                              ("pair", args.pair),
+                                              ("deadline_ms", args.deadline_ms)))
+        if same and not args.learned:
             line["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
-            line["roofline"]["traffic_source"] = "profiles/scs_epoch_c2_traffic.json"
-            line["roofline"]["algorithmic_bytes_per_launch"] = int(
-                edges * BYTES_PER_EDGE / max(1, epochs))
-    line["kl_sum"] = int(sum(r.kl for r in all_res))
-    line["kcap"] = int(s1["kcap"])
-    E.close()
+            line["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
+    by_tag = {r.tag: r for r in res}
 
-    # the same queries in the learned order (batched), and the plain order
-    # one query at a time
+    def agree(other):
+        both = [(by_tag[r.tag], r) for r in other
+                if r.status in PROVEN and r.tag in by_tag and by_tag[r.tag].status in PROVEN]
+        return len(both), sum(a.kl == b.kl for a, b in both)
+
+    # the same timed queries in the learned order (batched)
     if model is not None and not args.learned:
         print("[bench] learned leg", file=sys.stderr, flush=True)
         EL = scs.Engine(G, **eng_kw)
         EL.set_model(model, True)
-        EL.run([queries[0]])
+        EL.run([allq[0]])
         t0 = time.perf_counter()
-        lres = EL.run(queries, progress=10.0)
+        lres = run_batches(EL, timed, progress=20.0)
         lw = time.perf_counter() - t0
         EL.close()
-        same = all(a.kl == b.kl for a, b in zip(all_res, lres)
-                   if a.status != "CAPPED" and b.status != "CAPPED")
-        llat = [r.latency_us / 1000.0 for r in lres]
-        line["learned_batched"] = {
-            "value": round(len(queries) / lw, 3), "unit": "queries/s",
-            "p50_ms": round(percentile(llat, 50), 3), "p99_ms": round(percentile(llat, 99), 3),
-            "kl_sum": int(sum(r.kl for r in lres)),
-            "proven": int(sum(r.status in ("OK", "HEU_OPT", "INFEASIBLE") for r in lres)),
-            "optimum_equal_where_both_complete": same, "model": os.path.basename(args.model)}
+        ls = summary(lres, lw)
+        nb, ne = agree(lres)
+        ls.update({"model": os.path.relpath(mpath, ROOT), "both_proven": nb,
+                   "optimum_equal": ne == nb})
+        line["learned_batched"] = ls
+        line["learned_vs_plain_batched"] = round(ls["value"] / max(1e-9, main_sum["value"]), 3)
+    # the plain order one query at a time (one resident query), on a sample
+    # of the timed queries, against the batched rate
     if args.single_sample > 0:
         print("[bench] one-query-at-a-time leg", file=sys.stderr, flush=True)
-        sq = queries[:args.single_sample]
+        sq = timed_flat[:args.single_sample]
         ES = scs.Engine(G, **dict(eng_kw, limit_queries=1))
-        ES.run([queries[0]])
+        ES.run([allq[0]])
         t0 = time.perf_counter()
-        sres = ES.run(sq, progress=10.0)
+        sres = ES.run(sq, tags=list(range(len(sq))), progress=20.0)
         sw = time.perf_counter() - t0
         ES.close()
-        slat = [r.latency_us / 1000.0 for r in sres]
-        line["plain_one_at_a_time"] = {
-            "value": round(len(sq) / sw, 3), "unit": "queries/s", "sample": len(sq),
-            "p99_ms": round(percentile(slat, 99), 3)}
-        line["batched_vs_one_at_a_time"] = round(qps / (len(sq) / sw), 2)
-        if "learned_batched" in line:
-            line["learned_batched_vs_one_at_a_time"] = round(
-                line["learned_batched"]["value"] / (len(sq) / sw), 2)
+        ss = summary(sres, sw)
+        ss["sample"] = len(sq)
+        nb, ne = agree(sres)
+        ss.update({"both_proven": nb, "optimum_equal": ne == nb})
+        line["plain_one_at_a_time"] = ss
+        if ss["value"] > 0:
+            line["batched_vs_one_at_a_time"] = round(main_sum["value"] / ss["value"], 2)
+            if "learned_batched" in line:
+                line["learned_batched_vs_one_at_a_time"] = round(
+                    line["learned_batched"]["value"] / ss["value"], 2)
 
     # batched preprocessing alone (no search kernel sharing the GPU): the
     # roofline of the preprocessing kernels
     if args.prep_sample > 0:
-        line["prep_standalone"] = prep_leg(scs, G, core, queries[:args.prep_sample])
+        line["prep_standalone"] = prep_leg(scs, G, core, allq[:args.prep_sample])
 
-    pb = line["prep"]["algorithmic_bytes"]
-    pu = line["prep"]["device_us"]
-    line["prep"]["achieved_GBs"] = round(pb / (pu * 1e-6) / 1e9, 2) if pu > 0 else 0.0
+    for name in [x for x in args.extra.split(",") if x]:
+        line[f"extra_{name.lower()}"] = extra_leg(scs, configs, name, args, eng_kw)
 
     if not args.no_cpu:
-        from oracle.oracle import OracleGraph
+        # the oracle (same rules and level order, same budget) on the timed
+        # queries in order, one core, until the time bound; its proven
+        # answers must equal the GPU's where both proved
+        from oracle.oracle import FLAG_LEVELS, FLAG_PAIR, FLAGS_ALL, OracleGraph
+        flags = FLAGS_ALL | (FLAG_LEVELS if args.levels else 0) | (FLAG_PAIR if args.pair else 0)
         og = OracleGraph.from_csr(g.n, g.off, g.adj)
-        done = 0
+        done = proven = both = equal = 0
         tc0 = time.perf_counter()
-        for q in queries:
-            og.solve(*q, node_cap=args.budget)
+        for i, q in enumerate(timed_flat):
+            o = og.solve(*q, flags=flags, node_cap=args.budget,
+                         time_limit=args.deadline_ms / 1000.0)
             done += 1
+            if o["status"] in PROVEN:
+                proven += 1
+                r = by_tag.get(i)
+                if r is not None and r.status in PROVEN:
+                    both += 1
+                    equal += int(r.kl == o["kl"])
             if done % 10 == 0:
                 print(f"[cpu] {done} queries, {time.perf_counter() - tc0:.1f} s", file=sys.stderr,
                       flush=True)
             if time.perf_counter() - tc0 > args.cpu_seconds:
                 break
         tc = time.perf_counter() - tc0
-        line["cpu_baseline"] = {"value": round(done / tc, 4), "unit": "queries/s", "cores": 1,
-                                "kind": "port",
-                                "sample": f"first {done} queries of the same stream, node budget "
-                                          f"{args.budget}, serial trail solver, {tc:.1f} s"}
+        line["cpu_baseline"] = {
+            "value": round(proven / tc, 4), "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": (f"first {done} timed queries in stream order, serial oracle (same rules, "
+                       f"node budget {args.budget}, {args.deadline_ms} ms per query), "
+                       f"{proven} proven in {tc:.1f} s"),
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+        line["oracle_check"] = {"both_proven": both, "equal": equal}
     G.close()
     out = json.dumps(line)
     print(out, flush=True)

@@ -107,7 +107,10 @@ typedef struct scs_params {
                               with min degree > kl and |H| <= h share at least
                               2(kl+1)-h neighbours in H (adjacent) or 2(kl+1)-h+2
                               (not adjacent); no counterpart in the reference */
-  uint32_t reserved[3];
+  uint32_t deadline_ms;    /* per-query wall-clock bound from submission (0 = none): a
+                              query still searching when it passes is stopped at its
+                              next node and reported CAPPED with its best witness */
+  uint32_t reserved[2];
 } scs_params;
 
 typedef struct scs_result {

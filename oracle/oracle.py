@@ -75,6 +75,10 @@ def lib():
         L.orc_solve.restype = C.c_int
         L.orc_solve.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.c_uint32, C.c_uint64, C.POINTER(OrcResult), u32p, C.c_uint32]
+        L.orc_solve_timed.restype = C.c_int
+        L.orc_solve_timed.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, C.c_uint32, C.c_uint64, C.c_double,
+                                      C.POINTER(OrcResult), u32p, C.c_uint32]
         L.orc_solve_plain.restype = C.c_uint32
         L.orc_solve_plain.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
                                       C.POINTER(C.c_uint64)]
@@ -151,11 +155,15 @@ class OracleGraph:
         return dict(K=K, kept=kept[:K].copy(), off=loff[:K + 1].copy(),
                     adj=ladj[:nnz.value].copy(), dist=ldist[:K].copy())
 
-    def solve(self, l, h, q, heu=1, limit_doms=8, flags=FLAGS_ALL, node_cap=1 << 40):
+    def solve(self, l, h, q, heu=1, limit_doms=8, flags=FLAGS_ALL, node_cap=1 << 40,
+              time_limit=0.0):
+        """time_limit: wall-clock seconds after which the search ends CAPPED
+        (0 = none)."""
         r = OrcResult()
         wcap = max(h, 1) + self.n
         w = np.zeros(wcap, np.uint32)
-        lib().orc_solve(self.h, l, h, q, heu, limit_doms, flags, node_cap, C.byref(r), w, wcap)
+        lib().orc_solve_timed(self.h, l, h, q, heu, limit_doms, flags, node_cap, float(time_limit),
+                              C.byref(r), w, wcap)
         return dict(kl=r.kl, heu_kl=r.heu_kl, ku=r.ku, status=STATUS[r.status], nodes=r.nodes,
                     K=r.K, ubD=r.ubD, levels=r.levels, witness=w[:min(r.wsize, wcap)].copy())
 

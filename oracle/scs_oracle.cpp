@@ -23,6 +23,7 @@
 // Reference citations are file:line in wajidmanzoor/SCS.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -492,6 +493,7 @@ struct Params {
   u32 use_doms;
   u32 pair = 0;
   u32 root_filter = 0;  // level roots start with R = vertices that can matter at kl
+  double deadline = 0;  // steady-clock seconds after which the search stops (0 = none)
   u64 node_cap;
   u32 dmax;
   u32 ku;
@@ -499,6 +501,11 @@ struct Params {
 
 enum { ST_R = 0, ST_C = 1, ST_X = 2 };
 enum { NODE_PRUNE = 0, NODE_BRANCH = 1 };
+
+static double steady_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+static bool past(const Params& P) { return P.deadline > 0 && steady_s() > P.deadline; }
 
 // Level root filter: with levels on, a level's root starts with R = the
 // kept vertices other than q with core > kl and dist <= ubd_node(kl, h);
@@ -730,7 +737,7 @@ struct RecomputeSolver {
     std::vector<Task> kids;
     while (!stack.empty()) {
       if (real >= P.ku) return 0;
-      if (nodes >= P.node_cap) return 1;
+      if (nodes >= P.node_cap || past(P)) return 1;
       Task t = std::move(stack.back());
       stack.pop_back();
       kids.clear();
@@ -878,6 +885,15 @@ struct TrailSolver {
     return removed;
   }
 
+  // Bounds on the pair rule's work, so that a node whose C holds hubs of a
+  // large kept set stays cheap: a C vertex whose live neighbours' lists hold
+  // more than PAIR_MAX_SCAN entries, or more than what is left of the node's
+  // PAIR_NODE_SCAN entries, is skipped (a skipped check only weakens the
+  // pruning).
+  static constexpr u64 PAIR_MAX_SCAN = 1u << 20;
+  static constexpr u64 PAIR_NODE_SCAN = 1u << 24;
+  u64 pair_left = 0;
+
   // Pair rule.  Two vertices u, v of an H with min degree >= k = kl+1 and
   // |H| <= h each have >= k - [u~v] neighbours in H \ {u, v}, a set of at
   // most h-2 vertices, so they share >= 2k - h (adjacent) or 2k - h + 2 (not
@@ -900,6 +916,13 @@ struct TrailSolver {
       if (st[v] == ST_R) prl.push_back(v);
     for (size_t ci = 0; ci < Clist.size(); ci++) {
       const u32 u = Clist[ci];
+      u64 scan = 0;
+      for (u32 j = L.off[u]; j < L.off[u + 1]; j++) {
+        const u32 w = L.adj[j];
+        if (st[w] != ST_X) scan += L.off[w + 1] - L.off[w];
+      }
+      if (scan > PAIR_MAX_SCAN || scan > pair_left) continue;
+      pair_left -= scan;
       ptouch.clear();
       for (u32 j = L.off[u]; j < L.off[u + 1]; j++) padj[L.adj[j]] = 1;
       for (u32 j = L.off[u]; j < L.off[u + 1]; j++) {
@@ -941,6 +964,7 @@ struct TrailSolver {
     nodes++;
     const u32 h = P.h;
     bool full = (kl != kl_seen);
+    pair_left = PAIR_NODE_SCAN;
     while (true) {
       u32 Cs = (u32)Clist.size();
       if (Cs >= P.l && Cs <= h) {
@@ -1080,7 +1104,7 @@ struct TrailSolver {
     u32 from = 0, kl_seen = UINT32_MAX;
     while (true) {
       if (real >= P.ku) return 0;
-      if (nodes >= P.node_cap) return 1;
+      if (nodes >= P.node_cap || past(P)) return 1;
       int r = process(from, kl_seen);
       if (r == NODE_BRANCH) {
         Frame f;
@@ -1289,11 +1313,26 @@ struct orc_result {
   u32 levels;   // level searches run (1 without levels)
 };
 
+static int solve_impl(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 flags,
+                      u64 node_cap, double seconds, orc_result* res, u32* wit, u32 wcap);
+
 // flags bit0 red1, bit1 red2, bit2 red3, bit3 prun1, bit4 prun2, bit5 doms,
 // bit6 selects the recompute solver instead of the trail solver, bit7 levels,
 // bit8 pair rule (trail solver only)
 int orc_solve(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 flags,
               u64 node_cap, orc_result* res, u32* wit, u32 wcap) {
+  return solve_impl(o, l, h, q, heu, limit_doms, flags, node_cap, 0.0, res, wit, wcap);
+}
+// The same with a wall-clock limit (seconds, 0 = none): a search still
+// running when it passes ends CAPPED like one that reaches node_cap.
+int orc_solve_timed(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 flags,
+                    u64 node_cap, double seconds, orc_result* res, u32* wit, u32 wcap) {
+  return solve_impl(o, l, h, q, heu, limit_doms, flags, node_cap, seconds, res, wit, wcap);
+}
+}  // extern "C"
+
+static int solve_impl(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 flags,
+                      u64 node_cap, double seconds, orc_result* res, u32* wit, u32 wcap) {
   memset(res, 0, sizeof(*res));
   const Graph& g = o->g;
   if (h == 0 || q >= g.n || limit_doms > 32) {
@@ -1344,6 +1383,7 @@ int orc_solve(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 fl
   P.node_cap = node_cap;
   P.dmax = g.dmax;
   P.ku = ku;
+  P.deadline = seconds > 0 ? steady_s() + seconds : 0;
   // Levels (flag bit 7): search for min degree > flr with flr = ku-1 first,
   // then lower floors, down to the heuristic's kl.  A level whose search
   // ends without a solution above its floor proves opt <= flr; the first
@@ -1406,6 +1446,7 @@ int orc_solve(orc_graph* o, u32 l, u32 h, u32 q, u32 heu, u32 limit_doms, u32 fl
   return 0;
 }
 
+extern "C" {
 // Independent plain solver (no rules, no domination); returns the optimum
 // over the same kept set K (kl0 = 0, ubD = h-1 so K is q's component
 // restricted to core > 0), or UINT32_MAX when the node cap is hit.

@@ -59,6 +59,7 @@ hipError_t scs_launch_ms_fill(const int32_t* adj, const MsUnit* units, u32 nunit
 hipError_t scs_launch_epoch(const EngineArgs* A, hipStream_t s);
 hipError_t scs_launch_slot_write(const QuerySlot* stage, const u32* idx, const u32* wit, u32 n,
                                  QuerySlot* slots, u32* witness, hipStream_t s);
+hipError_t scs_launch_expire(QuerySlot* slots, const u32* idx, u32 n, hipStream_t s);
 hipError_t scs_launch_admit(const EngineArgs* A, const u32* slots, u32 n, u32* failed,
                             hipStream_t s);
 hipError_t scs_launch_pool_init(u64* seq, u32 n, hipStream_t s);
@@ -306,6 +307,8 @@ struct HostSlot {
   u32 kl0 = 0, ku = 0, K = 0;
   u64 a_off = 0, a_len = 0, b_off = 0, b_len = 0;
   u64 need = 0;  // arena bytes known to be needed when admission was refused
+  bool admitted = false;  // root task pushed: the query is on the device
+  bool expired = false;   // stopped at its deadline
 };
 
 struct scs_engine {
@@ -912,6 +915,8 @@ static void release_slot(scs_engine* E, u32 si) {
   if (!ok) fprintf(stderr, "scs: arena release of slot %u refused (range not allocated)\n", si);
   hs.a_len = hs.b_len = 0;
   hs.used = false;
+  hs.admitted = false;
+  hs.expired = false;
 }
 
 // Results of the queries whose slots the last epoch marked done.  Runs
@@ -960,7 +965,29 @@ static int collect(scs_engine* E) {
   return SCS_OK;
 }
 
+// Stops the admitted queries that have passed the per-query deadline.
+static int expire_late(scs_engine* E) {
+  if (!E->p.deadline_ms) return SCS_OK;
+  const double now = now_us(), lim = 1000.0 * E->p.deadline_ms;
+  std::vector<u32> late;
+  for (u32 si = 0; si < E->p.limit_queries; si++) {
+    HostSlot& hs = E->slots[si];
+    if (hs.used && hs.admitted && !hs.expired && now - hs.pq.t_submit > lim) {
+      hs.expired = true;
+      late.push_back(si);
+    }
+  }
+  if (late.empty()) return SCS_OK;
+  HIPCHK(hipMemcpyAsync(E->d_admit, late.data(), sizeof(u32) * late.size(), hipMemcpyHostToDevice,
+                        E->s));
+  HIPCHK(scs_launch_expire(E->d_slots, E->d_admit, (u32)late.size(), E->s));
+  HIPCHK(hipStreamSynchronize(E->s));
+  return SCS_OK;
+}
+
 static int launch_epoch(scs_engine* E) {
+  int rc = expire_late(E);
+  if (rc) return rc;
   EngineGlobals gl;
   HIPCHK(hipMemcpy(&gl, E->d_glob, sizeof(gl), hipMemcpyDeviceToHost));
   gl.idle = 0;
@@ -1145,6 +1172,8 @@ extern "C" int scs_step(scs_engine* E) {
     for (u32 i = 0; i < na; i++) {
       const u32 si = admitted[i];
       if (!failed[i]) {
+        E->slots[si].admitted = true;
+        E->slots[si].expired = false;
         nok++;
         continue;
       }

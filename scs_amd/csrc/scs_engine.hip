@@ -878,8 +878,13 @@ __device__ bool tight_exists(const W& w, u32 kl) {
 // touches listed in front, N(u) marked in nmark); the node is pruned when
 // another C vertex falls short, else every R vertex that falls short is
 // removed as one batch before the next u.  Returns -1 (prune), 1 (removed
-// something) or 0.
-__device__ __forceinline__ int pair_step(W& w, u32 kl) {
+// something) or 0.  A C vertex whose live neighbours' lists hold more than
+// PAIR_MAX_SCAN entries, or more than what is left of the node's
+// PAIR_NODE_SCAN entries (*left), is skipped: a node whose C holds hubs of a
+// large kept set stays cheap (a skipped check only weakens the pruning).
+#define PAIR_MAX_SCAN (1ull << 20)
+#define PAIR_NODE_SCAN (1ull << 24)
+__device__ __forceinline__ int pair_step(W& w, u32 kl, u64& left) {
   Lds* L = w.L;
   const long tadj = 2L * (long)(kl + 1) - (long)w.h;
   if (tadj + 2 <= 0) return 0;
@@ -893,20 +898,27 @@ __device__ __forceinline__ int pair_step(W& w, u32 kl) {
       L->cnt = 0;
     }
     u32 nl = 0;
+    u64 scan = 0;
     for (u32 j0 = ub; j0 < ue; j0 += 64) {
       const u32 j = j0 + LANE;
       u32 x = 0;
       bool live = false;
       if (j < ue) {
         x = w.adj[j];
-        w.nmark[x] = 1;
         live = dec_live(w, x);
       }
       const u64 bmk = __ballot(live);
-      if (live) w.work2[nl + __popcll(bmk & lanes_below())] = x;
+      if (live) {
+        w.work2[nl + __popcll(bmk & lanes_below())] = x;
+        scan += w.off[x + 1] - w.off[x];
+      }
       nl += __popcll(bmk);
     }
     w.edges += ue - ub;
+    scan = wsum64(scan);
+    if (scan > PAIR_MAX_SCAN || scan > left) continue;
+    left -= scan;
+    for (u32 j = ub + LANE; j < ue; j += 64) w.nmark[w.adj[j]] = 1;
     wsync();
     const u32* lv = w.work2;
     scan_adj<u32>(
@@ -1092,6 +1104,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
   const u32 h = w.h;
   u32 kl = cur_kl(w);
   bool full = (kl != kl_seen);
+  u64 pair_left = PAIR_NODE_SCAN;
   while (true) {
     const u32 Cs = w.nC;
     if (Cs >= w.l && Cs <= h) {
@@ -1144,7 +1157,7 @@ __device__ NodeOut process(W& w, u32 from, u32 kl_seen, u64 nodeno, u64& tp, u32
     if (removed && (w.flags & QF_RED3) && tight_exists(w, kl)) continue;
     if (w.flags & QF_PAIR) {
       ph_switch(L, tp, ph, PH_PAIR);
-      const int pr = pair_step(w, kl);
+      const int pr = pair_step(w, kl, pair_left);
       ph_switch(L, tp, ph, PH_RULES);
       if (pr < 0) {
         o.klc = kl;
@@ -2081,6 +2094,21 @@ extern "C" hipError_t scs_launch_slot_write(const QuerySlot* stage, const u32* i
                                             hipStream_t s) {
   if (n == 0) return hipSuccess;
   scs_slot_write<<<n, 64, 0, s>>>(stage, idx, wit, n, slots, witness);
+  return hipGetLastError();
+}
+
+// Queries past their deadline: capped and stopped, so their waves abandon
+// them at the next node.  Runs on the engine stream before an epoch.
+__global__ void scs_expire(QuerySlot* slots, const u32* idx, u32 n) {
+  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    atomicExch(&slots[idx[i]].capped, 1u);
+    atomicExch(&slots[idx[i]].stop, 1u);
+  }
+}
+extern "C" hipError_t scs_launch_expire(QuerySlot* slots, const u32* idx, u32 n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  scs_expire<<<(n + 63) / 64, 64, 0, s>>>(slots, idx, n);
   return hipGetLastError();
 }
 

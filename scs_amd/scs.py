@@ -46,7 +46,8 @@ class Params(C.Structure):
                 ("deterministic", C.c_uint32), ("learned", C.c_uint32),
                 ("model", C.c_void_p), ("heuristic_threads", C.c_uint32),
                 ("admit_per_step", C.c_uint32), ("no_lds_bitmaps", C.c_uint32),
-                ("levels", C.c_uint32), ("pair", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("levels", C.c_uint32), ("pair", C.c_uint32), ("deadline_ms", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class Result(C.Structure):

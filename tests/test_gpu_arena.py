@@ -2,7 +2,8 @@
 in one device arena; when it is full a query waits in the queue and is
 admitted once resident queries finish, and a query whose local graph does
 not fit even the empty arena is rejected.  With an arena holding about two
-local graphs, every golden C1 query still returns its optimum."""
+local graphs, every golden C1 query the oracle proved still returns its
+optimum."""
 
 import json
 import os
@@ -23,7 +24,9 @@ def granule(b):
 
 
 def test_arena_pressure_requeues_and_matches_golden():
-    rows = json.load(open(os.path.join(GOLD, "golden.json")))["C1"]
+    rows = [r for r in json.load(open(os.path.join(GOLD, "golden.json")))["C1"]
+            if r["status"] != "CAPPED" and r["nodes"] <= 20000]
+    assert len(rows) >= 20
     g = configs.graph("C1")
     og = OracleGraph.from_csr(g.n, g.off, g.adj)
     need = []
@@ -34,7 +37,8 @@ def test_arena_pressure_requeues_and_matches_golden():
     budget = 2 * max(need) + 4096
     print("arena bytes", budget, "per-query", need, flush=True)
     G = scs.Graph.from_csr(g.off, g.adj)
-    E = scs.Engine(G, limit_queries=16, mem_budget=budget)
+    # proven golden queries only; the budget only bounds a regression
+    E = scs.Engine(G, limit_queries=16, mem_budget=budget, node_budget=5_000_000)
     res = E.run([tuple(r["query"]) for r in rows], progress=10.0)
     E.close()
     for r, row in zip(res, rows):

@@ -107,23 +107,23 @@ def test_learned_order_keeps_every_optimum():
 
 
 def test_trained_model_gain_over_plain_rule():
-    """The committed model (trained by scs_amd.learn on traces of training
-    queries drawn with seeds 100..199) against the plain connection score on
-    unseen C1 queries with the same node budget: identical optima where both
-    searches complete, and at least as good a total of found kl."""
+    """The committed C1 model (trained by scs_amd.learn on search traces of
+    training queries drawn with seeds 100..199, labelled by the optimal
+    communities of the queries the search proved) against the plain
+    connection score on the workload's own (unseen) queries at the same
+    node budget: identical optima wherever both searches complete, at least
+    as many queries proven, and fewer nodes on the queries both prove."""
     from scs_amd import configs, learn
     path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "models", "branching_c1.npz")
     m = scs.Model.load(path)
     g = configs.graph("C1")
     G = scs.Graph.from_csr(g.off, g.adj)
     core = G.core().astype(np.int64)
-    qs = [(q.l, q.h, q.q, q.heu, q.limit_doms) for q in configs.queries("C1", core, count=60)]
-    ev = learn.evaluate(G, qs, m, 20000, limit_queries=64)
+    qs = [(q.l, q.h, q.q, q.heu, q.limit_doms) for q in configs.queries("C1", core, count=100)]
+    ev = learn.evaluate(G, qs, m, 50000, limit_queries=512)
     G.close()
-    p, l = ev["plain"], ev["learned"]
-    for kp, kl_, sp, sl in zip(p["kl"], l["kl"], p["status"], l["status"]):
-        if sp != "CAPPED" and sl != "CAPPED":
-            assert kp == kl_
-    print("plain kl sum", sum(p["kl"]), "learned kl sum", sum(l["kl"]),
-          "proven", p["proven"], l["proven"])
-    assert sum(l["kl"]) >= sum(p["kl"])
+    p, l, b = ev["plain"], ev["learned"], ev["both_proven"]
+    print("proven plain", p["proven"], "learned", l["proven"], "both", b)
+    assert b["optimum_equal"] and b["queries"] >= 15
+    assert l["proven"] >= p["proven"]
+    assert b["nodes_learned"] < b["nodes_plain"]

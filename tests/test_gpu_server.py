@@ -18,11 +18,20 @@ CLIENT = os.path.join(ROOT, "scs_amd", "server", "scs_client")
 GOLD = os.path.join(ROOT, "tests", "golden")
 
 
-def test_server_round_trip(tmp_path):
+# the 11 arguments after the graph: this build's defaults, and the fixed
+# arguments of the reference's own runtime script (scripts/GPU_runtime.sh:22:
+# partitionSize 300000, bufferSize 100000 words, copyLimit 0.5, readLimit 1,
+# limitQueries 8, factor 100, all rules and bounds on)
+ARGS = {"defaults": ["0", "0", "0.5", "1", "8", "1", "1", "1", "1", "1", "1"],
+        "reference_script": ["300000", "100000", "0.5", "1", "8", "100", "1", "1", "1", "1", "1"]}
+
+
+@pytest.mark.parametrize("args", list(ARGS))
+def test_server_round_trip(tmp_path, args):
     gold = {tuple(r["query"]): r["kl"] for r in json.load(open(os.path.join(GOLD, "golden.json")))["100.txt"]}
     res_dir = tmp_path / "results"
-    srv = subprocess.Popen([SERVER, os.path.join(GOLD, "100.txt"), "0", "0", "0.5", "1", "8", "1",
-                            "1", "1", "1", "1", "1", "--results", str(res_dir), "--queue-key", "x"],
+    srv = subprocess.Popen([SERVER, os.path.join(GOLD, "100.txt"), *ARGS[args],
+                            "--results", str(res_dir), "--queue-key", "x"],
                            stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     try:
         time.sleep(1.0)

@@ -285,3 +285,21 @@ def test_c1_golden_reproduces_and_witnesses_hold():
             assert (o["kl"], o["status"], o["nodes"]) == (r["kl"], r["status"], r["nodes"]), q
             redo += 1
     assert redo >= 20
+
+
+def test_time_limit_ends_a_search_capped():
+    """The oracle's wall-clock limit ends a long search CAPPED with a valid
+    lower bound (a C1 query the golden file records as unproven)."""
+    import time
+    from scs_amd import configs
+    gold = json.load(open(os.path.join(GOLD, "golden.json")))
+    r = next(x for x in gold["C1"] if x["status"] == "CAPPED")
+    g = configs.graph("C1")
+    og = OracleGraph.from_csr(g.n, g.off, g.adj)
+    t = time.time()
+    o = og.solve(*r["query"], flags=gold["C1_search"]["flags"], time_limit=0.5)
+    assert time.time() - t < 5.0
+    assert o["status"] == "CAPPED" and o["heu_kl"] <= o["kl"] <= o["ku"]
+    q = r["query"]
+    if o["kl"] > 0:
+        check_witness(g.off, g.adj, q[2], q[0], q[1], o["kl"], o["witness"])
