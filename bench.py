@@ -145,14 +145,53 @@ def extra_leg(scs, configs, name, args, eng_kw):
         wall = time.perf_counter() - t0
         E.close()
         out = summary(res, wall)
-        both = [(r, row) for r, row in zip(res, rows)
-                if r.status in PROVEN and row["status"] != "CAPPED"]
-        out.update({"node_budget": kw["node_budget"], "golden_proven": gold["C1_search"]["proven"],
-                    "both_proven": len(both),
+
+        def golden_check(rr):
+            both = [(r, row) for r, row in zip(rr, rows)
+                    if r.status in PROVEN and row["status"] != "CAPPED"]
+            return {"golden_both_proven": len(both),
                     "equal_to_golden": sum(r.kl == row["kl"] for r, row in both),
-                    "above_golden_lower_bound": sum(
-                        r.kl >= row["kl"] for r, row in zip(res, rows)
-                        if row["status"] == "CAPPED" and r.status in PROVEN)})
+                    "at_or_above_golden_lower_bound": sum(
+                        r.kl >= row["kl"] for r, row in zip(rr, rows)
+                        if row["status"] == "CAPPED" and r.status in PROVEN)}
+        out.update({"node_budget": kw["node_budget"], "golden_proven": gold["C1_search"]["proven"]})
+        out.update(golden_check(res))
+        # the learned order on the same 100 queries, with the C1 model
+        mpath = os.path.join(ROOT, "models", "branching_c1.npz")
+        if os.path.exists(mpath):
+            EL = scs.Engine(G, **kw)
+            EL.set_model(scs.Model.load(mpath), True)
+            EL.run([qs[0]])
+            t0 = time.perf_counter()
+            lres = EL.run(qs, tags=list(range(len(qs))), progress=20.0)
+            lw = time.perf_counter() - t0
+            EL.close()
+            ls = summary(lres, lw)
+            ls.update(golden_check(lres))
+            both = [(a, b) for a, b in zip(res, lres) if a.status in PROVEN and b.status in PROVEN]
+            ls.update({"model": os.path.relpath(mpath, ROOT), "both_proven": len(both),
+                       "optimum_equal": all(a.kl == b.kl for a, b in both),
+                       "nodes_on_both_proven": {"plain": int(sum(a.nodes for a, _ in both)),
+                                                "learned": int(sum(b.nodes for _, b in both))}})
+            out["learned"] = ls
+            out["learned_vs_plain"] = round(ls["value"] / max(1e-9, out["value"]), 3)
+        # the plain order one query at a time on the first queries
+        sq = qs[:args.c1_single]
+        if sq:
+            ES = scs.Engine(G, **dict(kw, limit_queries=1))
+            ES.run([qs[0]])
+            t0 = time.perf_counter()
+            sres = ES.run(sq, tags=list(range(len(sq))), progress=20.0)
+            sw = time.perf_counter() - t0
+            ES.close()
+            ss = summary(sres, sw)
+            ss["sample"] = len(sq)
+            out["plain_one_at_a_time"] = ss
+            if ss["value"] > 0:
+                out["batched_vs_one_at_a_time"] = round(out["value"] / ss["value"], 2)
+                if "learned" in out:
+                    out["learned_batched_vs_one_at_a_time"] = round(
+                        out["learned"]["value"] / ss["value"], 2)
     else:
         B = args.batch
         qs = configs.queries(name, core, count=args.extra_batches * B)
@@ -205,6 +244,8 @@ def main():
                          "the golden file's node cap, optima checked against it) and/or C3 "
                          "(--extra-batches batches like the headline's)")
     ap.add_argument("--extra-batches", type=int, default=5)
+    ap.add_argument("--c1-single", type=int, default=10,
+                    help="C1 queries also run one at a time (plain order)")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibrate-mib", type=int, default=0,
                     help="only run a plain elementwise pass over this many MiB (read once, "

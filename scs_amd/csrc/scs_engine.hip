@@ -1546,41 +1546,79 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
   const u32* Lst = task + 4 + nC;
   const u32 K = w.K;
   // queued[], nmark[] and acc[] are all-zero between nodes (every node
-  // clears what it set), so only the status needs initialising
+  // clears what it set), so only the status needs initialising.  The O(K)
+  // passes below handle eight 64-vertex chunks per step, so each lane keeps
+  // eight loads in flight instead of waiting on one at a time.
+  const u32 nw = (K + 31) / 32;
+  u32 base = 0;
+  // one chunk of the pass that writes the X bitmap (when in use) and the R
+  // list (ascending ids) from the chunk's status values
+  auto compact = [&](u32 c0, bool in, bool isx, bool isr) {
+    if (w.bm) {
+      const u64 m = __ballot(in && isx);
+      if (LANE == 0) w.xb[c0 >> 5] = (u32)m;
+      if (LANE == 1 && (c0 >> 5) + 1 < nw) w.xb[(c0 >> 5) + 1] = (u32)(m >> 32);
+    }
+    const u64 bmk = __ballot(in && isr);
+    if (in && isr) {
+      const u32 p = base + __popcll(bmk & lanes_below());
+      w.rlist[p] = c0 + LANE;
+      w.pos[c0 + LANE] = p;
+    }
+    base += __popcll(bmk);
+  };
   if (form == 2) {
     // a level's root (C = {q}): R = the vertices that can lie in an H with
     // min degree > kl and |H| <= h (core > kl, within ubD(kl, h) of q); the
     // root's rule-1/2 closure would remove every other one
-    const u32 kl = cur_kl(w), ub = ubd_node(kl, w.h);
-    for (u32 v = LANE; v < K; v += 64)
-      w.vs[v].st = (w.core[v] > kl && w.dist[v] <= ub) ? ST_R : ST_X;
+    const u32 kl = cur_kl(w), ub = ubd_node(kl, w.h), q0 = C[0];
+    for (u32 v0 = 0; v0 < K; v0 += 512) {
+      u32 cr[8], ds[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const u32 v = min(v0 + 64u * k + LANE, K - 1);
+        cr[k] = w.core[v];
+        ds[k] = w.dist[v];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const u32 c0 = v0 + 64u * k;
+        if (c0 >= K) break;
+        const u32 v = c0 + LANE;
+        const bool in = v < K, isq = v == q0;
+        const bool keep = !isq && cr[k] > kl && ds[k] <= ub;
+        if (in) w.vs[v].st = isq ? ST_C : (keep ? ST_R : ST_X);
+        compact(c0, in, !isq && !keep, keep);
+      }
+    }
+    if (LANE == 0) w.clist[0] = q0;
   } else {
     const u8 fill = form ? ST_X : ST_R;
     for (u32 v = LANE; v < K; v += 64) w.vs[v].st = fill;
     wsync();
     for (u32 i = LANE; i < nL; i += 64) w.vs[Lst[i]].st = form ? ST_R : ST_X;
+    wsync();
+    for (u32 i = LANE; i < nC; i += 64) {
+      w.vs[C[i]].st = ST_C;
+      w.clist[i] = C[i];
+    }
+    wsync();
+    for (u32 v0 = 0; v0 < K; v0 += 512) {
+      u8 st[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) st[k] = w.vs[min(v0 + 64u * k + LANE, K - 1)].st;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const u32 c0 = v0 + 64u * k;
+        if (c0 >= K) break;
+        compact(c0, c0 + LANE < K, st[k] == ST_X, st[k] == ST_R);
+      }
+    }
   }
-  wsync();
-  for (u32 i = LANE; i < nC; i += 64) {
-    w.vs[C[i]].st = ST_C;
-    w.clist[i] = C[i];
-  }
+  if (w.bm)
+    for (u32 i = LANE; i < nw; i += 64) w.tb[i] = 0;
   for (u32 i = LANE; i < SCS_HCAP + 4; i += 64) w.L->hist[i] = 0;
   wsync();
-  build_bitmaps(w);
-  // R list by ballot compaction (ascending ids)
-  u32 base = 0;
-  for (u32 v0 = 0; v0 < K; v0 += 64) {
-    u32 v = v0 + LANE;
-    bool r = v < K && w.vs[v].st == ST_R;
-    u64 bm = __ballot(r);
-    if (r) {
-      u32 p = base + __popcll(bm & lanes_below());
-      w.rlist[p] = v;
-      w.pos[v] = p;
-    }
-    base += __popcll(bm);
-  }
   w.nR = base;
   w.nC = nC;
   wsync();
@@ -1590,9 +1628,22 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
     // from the local degree (one coalesced pass) and subtract what X and C
     // take away.  dR(v) = deg(v) - |N(v) & X| - |N(v) & C|, dC(v) = |N(v) & C|;
     // counters of X vertices are never read, so their values do not matter.
-    for (u32 v = LANE; v < K; v += 64) {
-      w.vs[v].dC = 0;
-      w.vs[v].dR = w.off[v + 1] - w.off[v];
+    for (u32 v0 = 0; v0 < K; v0 += 512) {
+      u32 a[8], b[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const u32 v = min(v0 + 64u * k + LANE, K - 1);
+        a[k] = w.off[v];
+        b[k] = w.off[v + 1];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const u32 v = v0 + 64u * k + LANE;
+        if (v < K) {
+          w.vs[v].dC = 0;
+          w.vs[v].dR = b[k] - a[k];
+        }
+      }
     }
     wsync();
     if (nL >= 8) {

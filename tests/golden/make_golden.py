@@ -10,7 +10,9 @@ the independent plain solver where that finishes.  C1 is solved with the
 reference's rules plus the descending level search and the pair rule (the
 same optimum, far fewer nodes), all 100 queries in parallel processes, each
 with node cap C1_CAP; a query the cap stops stays in the file as CAPPED with
-its lower bound.  Run: python tests/golden/make_golden.py [--jobs N]
+its lower bound.  A second pass (--retry-cap N) re-solves only those with a
+larger cap.  Run: python tests/golden/make_golden.py [--jobs N], then
+python tests/golden/make_golden.py --retry-cap 3000000
 """
 
 import json
@@ -74,6 +76,42 @@ def solve_c1(jobs):
     return rows
 
 
+_RETRY_CAP = [0]
+
+
+def _c1_retry(i):
+    q = _C1["qs"][i]
+    r = solve(_C1["og"], (q.l, q.h, q.q, q.heu, q.limit_doms), cap=_RETRY_CAP[0], flags=C1_FLAGS)
+    r["index"] = i
+    return r
+
+
+def retry_capped(jobs, cap):
+    """Second pass: the C1 entries still CAPPED in golden.json are solved again
+    with node cap `cap`; the proven ones replace their entries (each entry
+    records the cap it was solved with)."""
+    import time
+    from multiprocessing import Pool
+    path = os.path.join(HERE, "golden.json")
+    out = json.load(open(path))
+    todo = [r["index"] for r in out["C1"] if r["status"] == "CAPPED"]
+    _RETRY_CAP[0] = cap
+    t0 = time.time()
+    with Pool(jobs, initializer=_c1_init) as p:
+        rows = list(p.imap_unordered(_c1_retry, todo))
+    for r in rows:
+        r["node_cap"] = cap
+        if r["status"] != "CAPPED":
+            out["C1"][r["index"]] = r
+    out["C1_search"]["retry"] = {"node_cap": cap, "queries": len(todo), "jobs": jobs,
+                                 "wall_seconds": round(time.time() - t0, 1),
+                                 "proven": sum(r["status"] != "CAPPED" for r in rows)}
+    out["C1_search"]["proven"] = sum(r["status"] != "CAPPED" for r in out["C1"])
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print("retried", len(todo), "proven", out["C1_search"]["retry"]["proven"])
+
+
 def main(jobs=8):
     out = {}
     og = OracleGraph.load(os.path.join(HERE, "graph.txt"))
@@ -106,4 +144,10 @@ if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
-    main(ap.parse_args().jobs)
+    ap.add_argument("--retry-cap", type=int, default=0,
+                    help="only re-solve the C1 entries still CAPPED, with this node cap")
+    a = ap.parse_args()
+    if a.retry_cap:
+        retry_capped(a.jobs, a.retry_cap)
+    else:
+        main(a.jobs)
