@@ -6,11 +6,14 @@ n = 500,000, 8M edges, exponent 2.2) and its query stream (l in [5,50],
 h = l + [5,150], q over every vertex, heuristic 0/1 at random,
 limit_dominating_set in {4,8,16}).  A step is one batch of consecutive
 queries of the stream (64 by default) submitted together to the persistent
-engine (admission: heuristic + batched GPU preprocessing; search; results),
-each query searched until its optimum is proven or its node budget is spent
-(status CAPPED: kl is then only a lower bound).  Warm-up steps take the
-stream's first batches and the timed steps the batches after them, so no
-query is timed twice.
+engine (admission: heuristic + batched GPU preprocessing; search; results).
+Batches are fed continuously: the next one goes in as soon as at most
+inflight - 1 batches' worth of queries remain in the engine (4 batches = the
+engine's 256 query slots by default).  Each query is searched until its
+optimum is proven, its node budget is spent or its deadline (from
+submission) passes (status CAPPED: kl is then only a lower bound).  Warm-up
+steps take the stream's first batches and the timed steps the batches after
+them, so no query is timed twice.
 
 value = proven-optimal queries per second over the timed steps: CAPPED
 answers are excluded from the numerator and counted beside it, with the
@@ -81,22 +84,56 @@ def summary(res, wall):
             "status_counts": st, "nodes": int(sum(r.nodes for r in res)), "wall_s": round(wall, 3)}
 
 
-def run_batches(E, batches, progress=None):
-    """Each batch submitted at once and drained before the next; tags number
-    the queries across batches."""
+def run_batches(E, batches, progress=None, inflight=1):
+    """Submits the batches in order, each batch at once; tags number the
+    queries across batches.  inflight = 1 drains each batch before the next
+    goes in.  inflight = d submits the next batch as soon as at most d - 1
+    batches' worth of queries are still in the engine, so new queries are
+    admitted while the previous ones search (continuous batching)."""
+    total = sum(len(b) for b in batches)
+    res, i, tag = [], 0, 0
+    last = time.time()
+    while True:
+        while i < len(batches) and tag - len(res) <= (inflight - 1) * len(batches[i]):
+            for (l, h, q, heu, ld) in batches[i]:
+                E.submit(l, h, q, heu, ld, tag)
+                tag += 1
+            i += 1
+        left = E.step()
+        res.extend(E.poll())
+        if progress is not None and time.time() - last > progress:
+            last = time.time()
+            print(f"[bench] {len(res)}/{total} queries done", file=sys.stderr, flush=True)
+        if left == 0 and i == len(batches):
+            break
+    res.extend(E.poll())
+    res.sort(key=lambda r: r.tag)
+    return res
+
+
+def run_one_at_a_time(E, queries, progress=None):
+    """Each query submitted only after the previous one has returned, so its
+    deadline runs while it alone is on the GPU."""
     out = []
-    for b in batches:
-        base = len(out)
-        out.extend(E.run(b, tags=list(range(base, base + len(b))), progress=progress))
+    last = time.time()
+    for i, qq in enumerate(queries):
+        out.extend(E.run([qq], tags=[i]))
+        if progress is not None and time.time() - last > progress:
+            last = time.time()
+            print(f"[bench] {len(out)}/{len(queries)} queries done", file=sys.stderr, flush=True)
     return out
 
 
-def prep_leg(scs, G, core, queries):
-    """Batched preprocessing of the queries alone (64 per batch, local graphs
-    built on the device, then released), with no search kernel on the GPU:
-    device time from HIP events and the kernels' algorithmic-byte count."""
-    print("[bench] standalone preprocessing leg", file=sys.stderr, flush=True)
-    EP = scs.Engine(G, limit_queries=64, waves_per_cu=1, mem_budget=32 << 30)
+def prep_leg(scs, G, core, queries, whole=1):
+    """Batched preprocessing of the queries alone (64 per batch, built on the
+    device, then released), with no search kernel on the GPU: device time
+    from HIP events and the kernels' algorithmic-byte count.  whole=1 is the
+    engine's own admission (queries keeping at least half the graph get a
+    distance array over the resident graph); whole=0 builds every query's
+    compacted local CSR."""
+    print(f"[bench] standalone preprocessing leg (whole_graph={whole})", file=sys.stderr,
+          flush=True)
+    EP = scs.Engine(G, limit_queries=64, waves_per_cu=1, mem_budget=32 << 30, whole_graph=whole)
     core64 = core.astype(np.int64)
     todo = []
     for (l, h, q, heu, ld) in queries:
@@ -181,7 +218,7 @@ def extra_leg(scs, configs, name, args, eng_kw):
             ES = scs.Engine(G, **dict(kw, limit_queries=1))
             ES.run([qs[0]])
             t0 = time.perf_counter()
-            sres = ES.run(sq, tags=list(range(len(sq))), progress=20.0)
+            sres = run_one_at_a_time(ES, sq, progress=20.0)
             sw = time.perf_counter() - t0
             ES.close()
             ss = summary(sres, sw)
@@ -200,7 +237,7 @@ def extra_leg(scs, configs, name, args, eng_kw):
         E = scs.Engine(G, **eng_kw)
         E.run([batches[0][0]])
         t0 = time.perf_counter()
-        res = run_batches(E, batches, progress=20.0)
+        res = run_batches(E, batches, progress=20.0, inflight=args.inflight)
         wall = time.perf_counter() - t0
         E.close()
         out = summary(res, wall)
@@ -216,6 +253,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="C4")
     ap.add_argument("--batch", type=int, default=64, help="queries per step")
+    ap.add_argument("--inflight", type=int, default=4,
+                    help="batches in the engine at once (1 = each batch drained before the next)")
     ap.add_argument("--budget", type=int, default=20000, help="node budget per query")
     ap.add_argument("--deadline-ms", type=int, default=2000,
                     help="per-query wall-clock bound from submission (CAPPED past it)")
@@ -237,6 +276,8 @@ def main():
                     help="timed queries also run one at a time (plain order)")
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prep-modes", default="1,0",
+                    help="--prep-only legs: 1 = as admitted (whole-graph queries), 0 = local CSRs")
     ap.add_argument("--prep-only", action="store_true",
                     help="only run the standalone batched-preprocessing leg (--prep-sample queries)")
     ap.add_argument("--extra", default="C1,C3",
@@ -244,7 +285,7 @@ def main():
                          "the golden file's node cap, optima checked against it) and/or C3 "
                          "(--extra-batches batches like the headline's)")
     ap.add_argument("--extra-batches", type=int, default=5)
-    ap.add_argument("--c1-single", type=int, default=10,
+    ap.add_argument("--c1-single", type=int, default=100,
                     help="C1 queries also run one at a time (plain order)")
     ap.add_argument("--out", default=None, help="also write the JSON line to this file")
     ap.add_argument("--copy-calibrate-mib", type=int, default=0,
@@ -292,8 +333,14 @@ def main():
                   heuristic_threads=args.heuristic_threads, admit_per_step=args.admit_per_step,
                   levels=args.levels, pair=args.pair, deadline_ms=args.deadline_ms)
     if args.prep_only:
-        print(json.dumps({"workload": args.config, "prep_standalone": prep_leg(
-            scs, G, core, allq[:args.prep_sample or len(allq)])}), flush=True)
+        sample = allq[:args.prep_sample or len(allq)]
+        out = {"workload": args.config}
+        modes = [int(x) for x in args.prep_modes.split(",") if x]
+        if 1 in modes:
+            out["prep_standalone"] = prep_leg(scs, G, core, sample)
+        if 0 in modes:
+            out["prep_standalone_local"] = prep_leg(scs, G, core, sample, whole=0)
+        print(json.dumps(out), flush=True)
         return
     E = scs.Engine(G, **eng_kw)
     if args.learned:
@@ -304,12 +351,12 @@ def main():
     E.run([allq[0]])
     if warm:
         print(f"[bench] warmup: {args.warmup} batches of {B}", file=sys.stderr, flush=True)
-        run_batches(E, warm)
+        run_batches(E, warm, inflight=args.inflight)
 
     s0 = E.stats()
     print(f"[bench] timed: {args.steps} batches of {B}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    res = run_batches(E, timed, progress=20.0)
+    res = run_batches(E, timed, progress=20.0, inflight=args.inflight)
     t1 = time.perf_counter()
     s1 = E.stats()
     E.close()
@@ -343,7 +390,7 @@ def main():
                    "node_budget": args.budget, "deadline_ms": args.deadline_ms,
                    "learned": bool(args.learned),
                    "levels": args.levels, "pair": args.pair, "waves": s1["waves"],
-                   "limit_queries": args.limit_queries},
+                   "limit_queries": args.limit_queries, "inflight": args.inflight},
         "proven": main_sum["proven"],
         "queries": main_sum["queries"],
         "proven_optimal_fraction": main_sum["proven_optimal_fraction"],
@@ -379,7 +426,8 @@ def main():
                                               ("node_budget", args.budget),
                                               ("batch", B), ("levels", args.levels),
                                               ("pair", args.pair),
-                                              ("deadline_ms", args.deadline_ms)))
+                                              ("deadline_ms", args.deadline_ms),
+                                              ("inflight", args.inflight)))
         if same and not args.learned:
             line["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
             line["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
@@ -397,7 +445,7 @@ def main():
         EL.set_model(model, True)
         EL.run([allq[0]])
         t0 = time.perf_counter()
-        lres = run_batches(EL, timed, progress=20.0)
+        lres = run_batches(EL, timed, progress=20.0, inflight=args.inflight)
         lw = time.perf_counter() - t0
         EL.close()
         ls = summary(lres, lw)
@@ -414,7 +462,7 @@ def main():
         ES = scs.Engine(G, **dict(eng_kw, limit_queries=1))
         ES.run([allq[0]])
         t0 = time.perf_counter()
-        sres = ES.run(sq, tags=list(range(len(sq))), progress=20.0)
+        sres = run_one_at_a_time(ES, sq, progress=20.0)
         sw = time.perf_counter() - t0
         ES.close()
         ss = summary(sres, sw)
@@ -432,6 +480,7 @@ def main():
     # roofline of the preprocessing kernels
     if args.prep_sample > 0:
         line["prep_standalone"] = prep_leg(scs, G, core, allq[:args.prep_sample])
+        line["prep_standalone_local"] = prep_leg(scs, G, core, allq[:args.prep_sample], whole=0)
 
     for name in [x for x in args.extra.split(",") if x]:
         line[f"extra_{name.lower()}"] = extra_leg(scs, configs, name, args, eng_kw)

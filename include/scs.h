@@ -110,7 +110,14 @@ typedef struct scs_params {
   uint32_t deadline_ms;    /* per-query wall-clock bound from submission (0 = none): a
                               query still searching when it passes is stopped at its
                               next node and reported CAPPED with its best witness */
-  uint32_t reserved[2];
+  uint32_t whole_graph;    /* how a query's graph reaches the search: 0 = always its own
+                              compacted CSR (the reference's CompressTask/NeighborUpdate
+                              form), 1 = the resident graph's CSR with the vertices
+                              outside the kept set marked removed, when the kept set
+                              holds at least half the graph (default), 2 = the resident
+                              CSR whenever the search state fits; same optimum, node
+                              counts and witness in every mode */
+  uint32_t reserved[1];
 } scs_params;
 
 typedef struct scs_result {

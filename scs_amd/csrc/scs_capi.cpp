@@ -44,6 +44,8 @@ hipError_t scs_launch_ms_count(u32 n, u32 G, const u64* vis, u64 valid, u32* gcn
                                unsigned long long* bytes, hipStream_t s);
 hipError_t scs_launch_ms_bscan(u32 nb, u32 G, u32* gcnt, u32* tot, unsigned long long* bytes,
                                hipStream_t s);
+hipError_t scs_launch_ms_gdist(u32 n, u32 G, const u64* vis, u64 gmask, const u16* dist16,
+                               const MsBatch* B, unsigned long long* bytes, hipStream_t s);
 hipError_t scs_launch_ms_scatter(u32 n, u32 G, const u64* vis, u64 valid, const u32* gcnt,
                                  const u32* core, const u16* dist16, const MsBatch* B, u32* locid,
                                  u32* qloc, unsigned long long* bytes, hipStream_t s);
@@ -88,6 +90,11 @@ struct scs_graph {
   i64* d_off = nullptr;
   int32_t* d_adj = nullptr;
   u32* d_core = nullptr;
+  // the search's view of the whole graph (queries that keep most of it
+  // search it in place): 32-bit offsets and the identity id map; null when
+  // the adjacency has 2^32 entries or more
+  u32* d_off32 = nullptr;
+  u32* d_ident = nullptr;
   double core_us = 0;
 };
 
@@ -163,6 +170,15 @@ static int graph_upload(scs_graph* G) {
     HIPCHK(hipMemcpy(G->d_adj, G->hg.adj.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
   G->hg.dmax = 0;
   for (u32 v = 0; v < n; v++) G->hg.dmax = std::max<u32>(G->hg.dmax, (u32)(G->hg.off[v + 1] - G->hg.off[v]));
+  if (nnz <= 0xffffffffull) {
+    std::vector<u32> o32(n + 1), id(std::max<u32>(n, 1));
+    for (u32 v = 0; v <= n; v++) o32[v] = (u32)G->hg.off[v];
+    for (u32 v = 0; v < n; v++) id[v] = v;
+    HIPCHK(hipMalloc(&G->d_off32, sizeof(u32) * (n + 1)));
+    HIPCHK(hipMalloc(&G->d_ident, sizeof(u32) * id.size()));
+    HIPCHK(hipMemcpy(G->d_off32, o32.data(), sizeof(u32) * (n + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G->d_ident, id.data(), sizeof(u32) * id.size(), hipMemcpyHostToDevice));
+  }
   return core_decompose(G);
 }
 
@@ -288,6 +304,8 @@ extern "C" void scs_graph_destroy(scs_graph* g) {
   if (g->d_off) (void)hipFree(g->d_off);
   if (g->d_adj) (void)hipFree(g->d_adj);
   if (g->d_core) (void)hipFree(g->d_core);
+  if (g->d_off32) (void)hipFree(g->d_off32);
+  if (g->d_ident) (void)hipFree(g->d_ident);
   delete g;
 }
 
@@ -307,6 +325,7 @@ struct HostSlot {
   u32 kl0 = 0, ku = 0, K = 0;
   u64 a_off = 0, a_len = 0, b_off = 0, b_len = 0;
   u64 need = 0;  // arena bytes known to be needed when admission was refused
+  bool whole = false;     // searches the resident graph's CSR (no local graph)
   bool admitted = false;  // root task pushed: the query is on the device
   bool expired = false;   // stopped at its deadline
 };
@@ -395,6 +414,7 @@ extern "C" void scs_params_default(scs_params* p) {
   p->admit_per_step = 256;
   p->levels = 1;
   p->pair = 1;
+  p->whole_graph = 1;
 }
 
 static u64 align256(u64 x) { return (x + 255) & ~255ull; }
@@ -418,6 +438,7 @@ static WaveLayout make_layout(u32 kcap) {
   L.clist = o; o = align256(o + 4 * (SCS_HCAP + 64));
   L.hist = o; o = align256(o + 4 * (SCS_HCAP + 8));
   L.hdr = o; o = align256(o + sizeof(WaveHdr));
+  L.gbits = o; o = align256(o + 8 * ((k4 + 63) / 32));
   L.stride = o;
   return L;
 }
@@ -752,11 +773,22 @@ static void fill_slot(scs_engine* E, u32 si, const MsQuery& P, u32 qloc) {
   Q.lvl_nodes0 = 0;
   Q.pending = 1;
   Q.node_budget = p.node_budget;
-  Q.off = P.loff;
-  Q.adj = P.ladj;
   Q.dist = P.ldist;
-  Q.orig = P.lorig;
-  Q.core = P.lcore;
+  if (hs.whole) {
+    // the resident graph with global ids: vertices outside the kept set
+    // have no distance and start removed
+    Q.flags |= QF_WHOLE;
+    Q.K = E->g->hg.n;
+    Q.off = E->g->d_off32;
+    Q.adj = (const u32*)E->g->d_adj;
+    Q.orig = E->g->d_ident;
+    Q.core = E->g->d_core;
+  } else {
+    Q.off = P.loff;
+    Q.adj = P.ladj;
+    Q.orig = P.lorig;
+    Q.core = P.lcore;
+  }
   Q.witness = E->d_witness + (u64)si * SCS_MAX_H;
 }
 
@@ -765,7 +797,7 @@ static void fill_slot(scs_engine* E, u32 si, const MsQuery& P, u32 qloc) {
 // SCS_EBIG (kept set above kcap) or SCS_EAGAIN (arena full; hs.need holds
 // the bytes known to be needed).  Successful slots get their QuerySlot.
 static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32* ubds, u32 cnt,
-                      int* rcs) {
+                      int* rcs, u32 whole) {
   if (cnt == 0 || cnt > 64) return SCS_EINVAL;
   const scs::HostGraph& hg = E->g->hg;
   const u32 n = hg.n;
@@ -832,15 +864,33 @@ static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32*
   HIPCHK(scs_launch_ms_bscan(cnt, G, E->d_gcnt, E->d_tot, E->d_prepbytes, s));
   HIPCHK(hipMemcpyAsync(E->h_tot, E->d_tot, sizeof(u32) * 64, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  u64 valid = 0;
+  u64 valid = 0, gmask = 0;
   u32 maxK = 0;
+  // whole-graph queries search the resident CSR in place: their search
+  // state covers every vertex, so the graph must fit the per-wave capacity
+  const bool can_whole = whole && E->g->d_off32 && n <= E->kcap;
   for (u32 b = 0; b < cnt; b++) {
     const u32 i = ord[b];
     HostSlot& hs = E->slots[sis[i]];
     const u32 K = E->h_tot[b];
     hs.K = K;
+    hs.whole = false;
     if (K > E->kcap || K == 0) {
       rcs[i] = SCS_EBIG;
+      continue;
+    }
+    if (can_whole && (whole == 2 || 2ull * K >= n)) {
+      // one distance array over the whole graph
+      const u64 a_len = sizeof(u32) * (u64)n;
+      if (!E->arena.alloc(a_len, &hs.a_off)) {
+        hs.need = scs::Arena::granule(a_len);
+        rcs[i] = SCS_EAGAIN;
+        continue;
+      }
+      hs.a_len = a_len;
+      hs.whole = true;
+      B.P[b] = MsQuery{nullptr, (u32*)(E->d_arena + hs.a_off), nullptr, nullptr, nullptr};
+      gmask |= 1ull << b;
       continue;
     }
     // block A: loff (K+1) | ldist (K) | lcore (K) | lorig (K).  a_len/b_len
@@ -861,8 +911,10 @@ static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32*
     valid |= 1ull << b;
     maxK = std::max(maxK, K);
   }
+  if (valid | gmask) HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
+  if (gmask)
+    HIPCHK(scs_launch_ms_gdist(n, G, E->d_vis, gmask, E->d_dist16, E->d_batch, E->d_prepbytes, s));
   if (valid) {
-    HIPCHK(hipMemcpyAsync(E->d_batch, &B, sizeof(MsBatch), hipMemcpyHostToDevice, s));
     HIPCHK(scs_launch_ms_scatter(n, G, E->d_vis, valid, E->d_gcnt, E->g->d_core, E->d_dist16,
                                  E->d_batch, E->d_locid64, E->d_tot + 128, E->d_prepbytes, s));
     HIPCHK(scs_launch_ms_degree(adj, E->d_units, E->nunits, E->d_hubs, E->nhubs, E->d_vis,
@@ -902,8 +954,10 @@ static int prep_batch(scs_engine* E, const u32* sis, const u32* kl0s, const u32*
   HIPCHK(hipEventElapsedTime(&ms, E->pe0, E->pe1));
   E->st.prep_us += ms * 1000.0;
   E->st.prep_batches++;
-  for (u32 b = 0; b < cnt; b++)
+  for (u32 b = 0; b < cnt; b++) {
     if ((valid >> b) & 1ull) fill_slot(E, sis[ord[b]], B.P[b], E->h_tot[128 + b]);
+    if ((gmask >> b) & 1ull) fill_slot(E, sis[ord[b]], B.P[b], B.q[b]);
+  }
   return SCS_OK;
 }
 
@@ -914,6 +968,7 @@ static void release_slot(scs_engine* E, u32 si) {
   if (hs.b_len) ok &= E->arena.release(hs.b_off, hs.b_len);
   if (!ok) fprintf(stderr, "scs: arena release of slot %u refused (range not allocated)\n", si);
   hs.a_len = hs.b_len = 0;
+  hs.whole = false;
   hs.used = false;
   hs.admitted = false;
   hs.expired = false;
@@ -943,8 +998,11 @@ static int collect(scs_engine* E) {
     int status;
     // kl is raised only after its witness is stored, so kl == wkl whenever
     // a witness exists; a lost witness (lock never acquired) leaves kl at the
-    // last stored one and the result is only a lower bound
-    if (Q.capped || Q.wlost || (Q.wsize && Q.wkl != Q.kl))
+    // last stored one and the result is only a lower bound.  A query stopped
+    // by its budget or deadline is still proven when kl reached the slot's
+    // ku (min(core[q], h-1), lowered only by levels that ended unstopped).
+    const bool wit_ok = !Q.wlost && !(Q.wsize && Q.wkl != Q.kl);
+    if (!wit_ok || (Q.capped && Q.kl < Q.ku))
       status = SCS_ST_CAPPED;
     else if (Q.kl == 0 && hs.pq.l >= 2)
       status = SCS_ST_INFEASIBLE;
@@ -1096,7 +1154,8 @@ extern "C" int scs_step(scs_engine* E) {
   std::vector<int> rcs(cand_i.size(), SCS_OK);
   for (size_t c0 = 0; c0 < cand_i.size(); c0 += 64) {
     const u32 cnt = (u32)std::min<size_t>(64, cand_i.size() - c0);
-    int rc = prep_batch(E, &cand_si[c0], &cand_kl0[c0], &cand_ubd[c0], cnt, &rcs[c0]);
+    int rc = prep_batch(E, &cand_si[c0], &cand_kl0[c0], &cand_ubd[c0], cnt, &rcs[c0],
+                        E->p.whole_graph);
     if (rc) return rc;
   }
   std::vector<PendingQuery> requeue;
@@ -1287,7 +1346,11 @@ extern "C" int scs_prep_probe_many(scs_engine* E, uint32_t count, const uint32_t
     hs.a_len = hs.b_len = 0;
   }
   std::vector<int> rcs(count, SCS_OK);
-  const int brc = prep_batch(E, sis.data(), kl0, ubD, count, rcs.data());
+  // local graphs when the caller asks for them; otherwise the batch is
+  // prepared as admission prepares it (whole-graph queries included)
+  const bool copy = kept || loff || dist || ladj;
+  const int brc = prep_batch(E, sis.data(), kl0, ubD, count, rcs.data(),
+                             copy ? 0u : E->p.whole_graph);
   int out = brc;
   for (u32 i = 0; i < count && brc == SCS_OK; i++) {
     HostSlot& hs = E->slots[sis[i]];
@@ -1295,6 +1358,7 @@ extern "C" int scs_prep_probe_many(scs_engine* E, uint32_t count, const uint32_t
     nnz[i] = 0;
     if (rcs[i] == SCS_EAGAIN) out = SCS_EAGAIN;
     if (rcs[i] != SCS_OK) continue;
+    if (hs.whole) continue;  // no local graph (nnz stays 0)
     const QuerySlot& Q = E->h_slots[sis[i]];
     const u32 k = hs.K;
     u32 nz = 0;

@@ -148,6 +148,7 @@ struct W {
   u32* sav;    // saved (vertex, dR) pairs of recount batches, as a stack
   Frame* frames;
   u32* clist;
+  u64* gb;     // R/C bitmaps of the last rebuild (WaveLayout::gbits)
   u32* ghist;  // HBM copy of the histogram between epochs
   WaveHdr* hdr;
   Lds* L;
@@ -190,6 +191,7 @@ __device__ void bind(W& w, const EngineArgs& A, u32 wave, Lds* L) {
   w.sav = (u32*)(base + A.lay.sav);
   w.frames = (Frame*)(base + A.lay.frames);
   w.clist = (u32*)(base + A.lay.clist);
+  w.gb = (u64*)(base + A.lay.gbits);
   w.ghist = (u32*)(base + A.lay.hist);
   w.hdr = (WaveHdr*)(base + A.lay.hdr);
   w.L = L;
@@ -1551,15 +1553,20 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
   // eight loads in flight instead of waiting on one at a time.
   const u32 nw = (K + 31) / 32;
   u32 base = 0;
-  // one chunk of the pass that writes the X bitmap (when in use) and the R
-  // list (ascending ids) from the chunk's status values
-  auto compact = [&](u32 c0, bool in, bool isx, bool isr) {
+  // one chunk of the pass that writes the X bitmap (when in use), the R and
+  // C bitmaps of gb and the R list (ascending ids) from the chunk's status
+  // values
+  auto compact = [&](u32 c0, bool in, bool isx, bool isr, bool isc) {
     if (w.bm) {
       const u64 m = __ballot(in && isx);
       if (LANE == 0) w.xb[c0 >> 5] = (u32)m;
       if (LANE == 1 && (c0 >> 5) + 1 < nw) w.xb[(c0 >> 5) + 1] = (u32)(m >> 32);
     }
     const u64 bmk = __ballot(in && isr);
+    const u64 cmk = __ballot(in && isc);
+    if (LANE == 0) w.gb[c0 >> 5] = (bmk & 0xFFFFFFFFull) | (cmk << 32);
+    if (LANE == 1 && (c0 >> 5) + 1 < nw)
+      w.gb[(c0 >> 5) + 1] = (bmk >> 32) | (cmk & 0xFFFFFFFF00000000ull);
     if (in && isr) {
       const u32 p = base + __popcll(bmk & lanes_below());
       w.rlist[p] = c0 + LANE;
@@ -1567,11 +1574,13 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
     }
     base += __popcll(bmk);
   };
-  if (form == 2) {
+  if (form >= 2) {
     // a level's root (C = {q}): R = the vertices that can lie in an H with
     // min degree > kl and |H| <= h (core > kl, within ubD(kl, h) of q); the
-    // root's rule-1/2 closure would remove every other one
-    const u32 kl = cur_kl(w), ub = ubd_node(kl, w.h), q0 = C[0];
+    // root's rule-1/2 closure would remove every other one.  A plain root of
+    // a whole-graph query (form 3): R = every other kept vertex (the ones
+    // with a distance)
+    const u32 kl = cur_kl(w), ub = form == 2 ? ubd_node(kl, w.h) : SCS_DIST_NONE - 1, q0 = C[0];
     for (u32 v0 = 0; v0 < K; v0 += 512) {
       u32 cr[8], ds[8];
 #pragma unroll
@@ -1586,9 +1595,9 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
         if (c0 >= K) break;
         const u32 v = c0 + LANE;
         const bool in = v < K, isq = v == q0;
-        const bool keep = !isq && cr[k] > kl && ds[k] <= ub;
+        const bool keep = !isq && (form == 3 || cr[k] > kl) && ds[k] <= ub;
         if (in) w.vs[v].st = isq ? ST_C : (keep ? ST_R : ST_X);
-        compact(c0, in, !isq && !keep, keep);
+        compact(c0, in, !isq && !keep, keep, in && isq);
       }
     }
     if (LANE == 0) w.clist[0] = q0;
@@ -1611,7 +1620,7 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
       for (int k = 0; k < 8; k++) {
         const u32 c0 = v0 + 64u * k;
         if (c0 >= K) break;
-        compact(c0, c0 + LANE < K, st[k] == ST_X, st[k] == ST_R);
+        compact(c0, c0 + LANE < K, st[k] == ST_X, st[k] == ST_R, st[k] == ST_C);
       }
     }
   }
@@ -1686,12 +1695,15 @@ __device__ __forceinline__ void rebuild(W& w, const u32* task) {
     return;
   }
   // the list holds R: count the live vertices' neighbours directly
-  // (dC in the high word, dR in the low word)
+  // (dC in the high word, dR in the low word).  Membership comes from the
+  // gb bitmaps (8 bytes per 32 vertices) rather than from the 16-byte state
+  // records, so the gathers land in a small array that stays in the caches.
   scan_adj<u64>(
       w, nR + nC, [&](u32 i) { return i < nR ? w.rlist[i] : w.clist[i - nR]; },
       [&](u32 x) {
-        const u8 s = w.vs[x].st;
-        return ((u64)(s == ST_C) << 32) | (u64)(s == ST_R);
+        const u64 g = w.gb[x >> 5];
+        const u32 b = x & 31;
+        return (((g >> (32 + b)) & 1ull) << 32) | ((g >> b) & 1ull);
       },
       [&](u32 i, u32 v, u64 cr) {
         const u32 c = (u32)(cr >> 32), r = (u32)cr;
@@ -2114,7 +2126,9 @@ __global__ void __launch_bounds__(64) scs_admit(EngineArgs A, const u32* slots_i
       dst[0] = sl;
       dst[1] = 1;
       dst[2] = 0;
-      dst[3] = (A.slots[sl].flags & QF_LEVELS) ? 2u : 0u;  // level root or plain root
+      // level root, plain root of a whole-graph query, or plain root
+      const u32 fl = A.slots[sl].flags;
+      dst[3] = (fl & QF_LEVELS) ? 2u : ((fl & QF_WHOLE) ? 3u : 0u);
       dst[4] = A.slots[sl].qloc;
       failed[i] = 0;
       __threadfence();

@@ -36,6 +36,10 @@ typedef int64_t i64;
 #define QF_LEARNED 64u
 #define QF_LEVELS 128u  // descending level floors
 #define QF_PAIR 256u    // common-neighbour pair rule
+// engine only: the query searches the resident graph's CSR (global ids), and
+// its distance array holds SCS_DIST_NONE for the vertices it does not keep
+#define QF_WHOLE 512u
+#define SCS_DIST_NONE 0xFFFFFFFFu
 
 // Per-query slot, device resident.  Written by the host at admission (while
 // no kernel touches the slot) and by search waves through atomics.
@@ -112,14 +116,17 @@ static_assert(__builtin_offsetof(VState, dR) == 0 && __builtin_offsetof(VState, 
               "VState word layout");
 
 // Offsets (bytes) of each per-wave array inside one wave's region.
+// gbits: one u64 per 32 local ids, R bits in the low word and C bits in the
+// high word, written by a task rebuild for its neighbour-count scan.
 struct WaveLayout {
-  u64 vs, nmark, pos, rlist, work, work2, front, aux, trail, sav, frames, clist, hist, hdr;
+  u64 vs, nmark, pos, rlist, work, work2, front, aux, trail, sav, frames, clist, hist, hdr, gbits;
   u64 stride;
 };
 
 // Pool of donated tasks: bounded MPMC ring (per-slot sequence numbers).
 // Task payload: [qslot, nC, nL, form(0 = list is X, 1 = list is R, 2 = a level root:
-// C = {q}, R = the kept vertices with core > kl within ubD(kl, h))], C[nC], L[nL].
+// C = {q}, R = the kept vertices with core > kl within ubD(kl, h), 3 = a plain root of a
+// whole-graph query: C = {q}, R = the other kept vertices)], C[nC], L[nL].
 struct Pool {
   u64* seq;        // nslots
   u32* data;       // nslots * slot_words

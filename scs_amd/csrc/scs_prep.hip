@@ -359,10 +359,15 @@ __global__ void __launch_bounds__(256) ms_scatter(u32 n, u32 G, const u64* __res
   const u32 wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const u32 nw = (gridDim.x * blockDim.x) >> 6;
   const u64 below = ms_below();
+  // lane b holds query b's output pointers and vertex; the per-query loop
+  // below takes them by shuffles, so its stores wait on no memory load
+  const MsQuery Qb = B->P[lane];
+  const u32 qb = B->q[lane];
   u64 nbytes = 0;
   for (u32 g = wid; g < G; g += nw) {
     const u64 v = (u64)g * 64 + lane;
     const u64 m = v < n ? vis[v] & valid : 0;
+    const u32 gb = gcnt[(u64)g * 64 + lane];  // lane b: query b's first id in the group
     u32 cv = 0;
     if (m) {
       cv = core[v];
@@ -373,15 +378,50 @@ __global__ void __launch_bounds__(256) ms_scatter(u32 n, u32 G, const u64* __res
       const int b = __ffsll((unsigned long long)any) - 1;
       any &= any - 1;
       const u64 bal = __ballot((m >> b) & 1ull);
+      const u32 base = (u32)__shfl((int)gb, b), qv = (u32)__shfl((int)qb, b);
+      u32* const lorig = (u32*)ms_shfl64((u64)Qb.lorig, b);
+      u32* const ldist = (u32*)ms_shfl64((u64)Qb.ldist, b);
+      u32* const lcore = (u32*)ms_shfl64((u64)Qb.lcore, b);
       if ((m >> b) & 1ull) {
-        const u32 id = gcnt[(u64)g * 64 + b] + (u32)__popcll(bal & below);
-        const MsQuery Q = B->P[b];
-        Q.lorig[id] = (u32)v;
-        Q.ldist[id] = dist16[v * MS_BITS + b];
-        Q.lcore[id] = cv;
+        const u32 id = base + (u32)__popcll(bal & below);
+        lorig[id] = (u32)v;
+        ldist[id] = dist16[v * MS_BITS + b];
+        lcore[id] = cv;
         locid[v * MS_BITS + b] = id;
-        if ((u32)v == B->q[b]) qloc[b] = id;
+        if ((u32)v == qv) qloc[b] = id;
         nbytes += 4 + 2 + 12 + 4;
+      }
+    }
+  }
+  ms_count_bytes(bytes, nbytes);
+}
+
+// Whole-graph queries (gmask): query b's distance array over every vertex of
+// the graph, SCS_DIST_NONE where b does not keep the vertex.  Lane = vertex,
+// so each query's 64 values of a group are one coalesced store.
+__global__ void __launch_bounds__(256) ms_gdist(u32 n, u32 G, const u64* __restrict__ vis,
+                                                u64 gmask, const u16* __restrict__ dist16,
+                                                const MsBatch* __restrict__ B,
+                                                unsigned long long* bytes) {
+  const u32 lane = threadIdx.x & 63;
+  const u32 wid = ms_wave_id();
+  const u32 nw = (gridDim.x * blockDim.x) >> 6;
+  const u64 dp = (u64)B->P[lane].ldist;
+  u64 nbytes = 0;
+  for (u32 g = wid; g < G; g += nw) {
+    const u64 v = (u64)g * 64 + lane;
+    const bool in = v < n;
+    const u64 m = in ? vis[v] & gmask : 0ull;
+    if (in) nbytes += 8;
+    u64 left = gmask;
+    while (left) {
+      const int b = __ffsll((unsigned long long)left) - 1;
+      left &= left - 1;
+      u32* const d = (u32*)ms_shfl64(dp, b);
+      if (in) {
+        const bool kept = (m >> b) & 1ull;
+        d[v] = kept ? (u32)dist16[v * MS_BITS + b] : SCS_DIST_NONE;
+        nbytes += kept ? 6 : 4;
       }
     }
   }
@@ -758,6 +798,12 @@ extern "C" hipError_t scs_launch_ms_count(u32 n, u32 G, const u64* vis, u64 vali
 extern "C" hipError_t scs_launch_ms_bscan(u32 nb, u32 G, u32* gcnt, u32* tot,
                                           unsigned long long* bytes, hipStream_t s) {
   ms_bscan<<<nb, 64, 0, s>>>(G, gcnt, tot, bytes);
+  return hipGetLastError();
+}
+extern "C" hipError_t scs_launch_ms_gdist(u32 n, u32 G, const u64* vis, u64 gmask,
+                                          const u16* dist16, const MsBatch* B,
+                                          unsigned long long* bytes, hipStream_t s) {
+  ms_gdist<<<ms_blocks((u64)G * 64, MS_GRID), 256, 0, s>>>(n, G, vis, gmask, dist16, B, bytes);
   return hipGetLastError();
 }
 extern "C" hipError_t scs_launch_ms_scatter(u32 n, u32 G, const u64* vis, u64 valid,

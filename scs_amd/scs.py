@@ -47,7 +47,7 @@ class Params(C.Structure):
                 ("model", C.c_void_p), ("heuristic_threads", C.c_uint32),
                 ("admit_per_step", C.c_uint32), ("no_lds_bitmaps", C.c_uint32),
                 ("levels", C.c_uint32), ("pair", C.c_uint32), ("deadline_ms", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("whole_graph", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
 class Result(C.Structure):

@@ -6,6 +6,7 @@
 //              [--wave-per-cu N] [--limit-tasks N] [--node-budget N]
 //              [--mem-budget BYTES] [--learned MODEL.bin] [--time-slice-us N]
 //              [--results DIR] [--queue-key C] [--levels 0|1] [--pair 0|1]
+//              [--whole-graph 0|1|2]
 //
 // The twelve positional arguments are those of the reference (main.cu:
 // 481-502).  A listener thread receives "<l> <h> <q> <heu> <limitDoms>"
@@ -100,6 +101,7 @@ int main(int argc, char** argv) {
     else if (a == "--time-slice-us") p.time_slice_us = (uint32_t)atoi(val());
     else if (a == "--levels") p.levels = (uint32_t)atoi(val());
     else if (a == "--pair") p.pair = (uint32_t)atoi(val());
+    else if (a == "--whole-graph") p.whole_graph = (uint32_t)atoi(val());
     else if (a == "--learned") model_path = val();
     else if (a == "--results") results_dir = val();
     else if (a == "--queue-key") key = val()[0];

@@ -97,19 +97,27 @@ def retry_capped(jobs, cap):
     todo = [r["index"] for r in out["C1"] if r["status"] == "CAPPED"]
     _RETRY_CAP[0] = cap
     t0 = time.time()
+    done = proven = 0
     with Pool(jobs, initializer=_c1_init) as p:
-        rows = list(p.imap_unordered(_c1_retry, todo))
-    for r in rows:
-        r["node_cap"] = cap
-        if r["status"] != "CAPPED":
-            out["C1"][r["index"]] = r
-    out["C1_search"]["retry"] = {"node_cap": cap, "queries": len(todo), "jobs": jobs,
-                                 "wall_seconds": round(time.time() - t0, 1),
-                                 "proven": sum(r["status"] != "CAPPED" for r in rows)}
-    out["C1_search"]["proven"] = sum(r["status"] != "CAPPED" for r in out["C1"])
-    with open(path, "w") as f:
-        json.dump(out, f, indent=1)
-    print("retried", len(todo), "proven", out["C1_search"]["retry"]["proven"])
+        # the file is rewritten after every answer, so a pass cut short keeps
+        # what it proved
+        for r in p.imap_unordered(_c1_retry, todo):
+            done += 1
+            r["node_cap"] = cap
+            if r["status"] != "CAPPED":
+                out["C1"][r["index"]] = r
+                proven += 1
+            out["C1_search"]["retry"] = {"node_cap": cap, "queries": len(todo), "answered": done,
+                                         "jobs": jobs, "wall_seconds": round(time.time() - t0, 1),
+                                         "proven": proven}
+            out["C1_search"]["proven"] = sum(x["status"] != "CAPPED" for x in out["C1"])
+            tmp = path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(out, f, indent=1)
+            os.replace(tmp, path)
+            print(f"retry {done}/{len(todo)}: index {r['index']} {r['status']} kl {r['kl']} "
+                  f"nodes {r['nodes']}", flush=True)
+    print("retried", len(todo), "proven", proven)
 
 
 def main(jobs=8):

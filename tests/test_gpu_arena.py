@@ -3,7 +3,9 @@ in one device arena; when it is full a query waits in the queue and is
 admitted once resident queries finish, and a query whose local graph does
 not fit even the empty arena is rejected.  With an arena holding about two
 local graphs, every golden C1 query the oracle proved still returns its
-optimum."""
+optimum.  These engines build every query's local graph (whole_graph=0); a
+whole-graph query needs only its 4n-byte distance array, which the small
+arena still holds."""
 
 import json
 import os
@@ -38,7 +40,8 @@ def test_arena_pressure_requeues_and_matches_golden():
     print("arena bytes", budget, "per-query", need, flush=True)
     G = scs.Graph.from_csr(g.off, g.adj)
     # proven golden queries only; the budget only bounds a regression
-    E = scs.Engine(G, limit_queries=16, mem_budget=budget, node_budget=5_000_000)
+    E = scs.Engine(G, limit_queries=16, mem_budget=budget, node_budget=5_000_000,
+                   whole_graph=0)
     res = E.run([tuple(r["query"]) for r in rows], progress=10.0)
     E.close()
     for r, row in zip(res, rows):
@@ -47,7 +50,7 @@ def test_arena_pressure_requeues_and_matches_golden():
 
     # an arena smaller than the smallest local graph rejects every query
     # that needs one, and the engine stays usable
-    E = scs.Engine(G, limit_queries=16, mem_budget=min(need) // 2)
+    E = scs.Engine(G, limit_queries=16, mem_budget=min(need) // 2, whole_graph=0)
     res = E.run([tuple(r["query"]) for r in rows[:3]])
     E.close()
     for r, row in zip(res, rows[:3]):
@@ -55,4 +58,12 @@ def test_arena_pressure_requeues_and_matches_golden():
             assert r.status == "HEU_OPT"
         else:
             assert r.status == "REJECTED" and r.heu_kl <= r.kl <= row["kl"]
+    # the same arena admits every query as a whole-graph query
+    assert min(need) // 2 >= granule(4 * g.n)
+    E = scs.Engine(G, limit_queries=16, mem_budget=min(need) // 2, whole_graph=2,
+                   node_budget=5_000_000)
+    res = E.run([tuple(r["query"]) for r in rows[:3]])
+    E.close()
+    for r, row in zip(res, rows[:3]):
+        assert r.status in ("OK", "HEU_OPT") and r.kl == row["kl"], row["query"]
     G.close()
