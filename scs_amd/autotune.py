@@ -19,8 +19,11 @@ metric: proven-optimal queries per second; ties (common when few queries are
 proven) go to the larger sum of the lower bounds found, then to the faster
 run.
 
+Queries are fed with continuous batching (the next batch enters while at
+most inflight - 1 batches remain), as the bench and the serving loop do.
+
 Usage:
-  python -m scs_amd.autotune --config C4 --batches 5 --batch 64 --budget 20000 \
+  python -m scs_amd.autotune --config C4 --batches 10 --batch 64 --budget 20000 \
       --deadline-ms 2000 --out scs_amd/tuned_c4.json
 """
 
@@ -59,31 +62,50 @@ def parse_grid(text):
 PROVEN = ("OK", "HEU_OPT", "INFEASIBLE")
 
 
-def run_once(G, batches, budget, deadline_ms, params, mem_budget):
+def feed(E, batches, inflight):
+    """Submits the batches in order, the next one as soon as at most
+    inflight - 1 batches' worth of queries remain in the engine (continuous
+    batching, as the serving loop and the bench feed it)."""
+    res, i, tag = [], 0, 0
+    while True:
+        while i < len(batches) and tag - len(res) <= (inflight - 1) * len(batches[i]):
+            for (l, h, q, heu, ld) in batches[i]:
+                E.submit(l, h, q, heu, ld, tag)
+                tag += 1
+            i += 1
+        left = E.step()
+        res.extend(E.poll())
+        if left == 0 and i == len(batches):
+            break
+    res.extend(E.poll())
+    return res
+
+
+def run_once(G, batches, budget, deadline_ms, params, mem_budget, inflight=4):
     E = scs.Engine(G, node_budget=budget, mem_budget=mem_budget, deadline_ms=deadline_ms,
                    **params)
     E.run([batches[0][0]])  # code objects and first-touch outside the timing
     s0 = E.stats()
     t0 = time.perf_counter()
-    res = []
-    for b in batches:
-        res.extend(E.run(b))
+    res = feed(E, batches, inflight)
     wall = time.perf_counter() - t0
     s1 = E.stats()
     E.close()
     proven = sum(r.status in PROVEN for r in res)
     return {"proven_qps": proven / wall, "qps": len(res) / wall, "wall_s": wall,
             "nodes": int(s1["nodes"] - s0["nodes"]), "proven": proven,
+            "rejected": sum(r.status == "REJECTED" for r in res),
             "kl_sum": int(sum(r.kl for r in res))}
 
 
-def tune(G, batches, budget, deadline_ms, grid=None, mem_budget=32 << 30, log=sys.stderr):
+def tune(G, batches, budget, deadline_ms, grid=None, mem_budget=32 << 30, log=sys.stderr,
+         inflight=4):
     grid = grid or GRID
     keys = list(grid)
     rows = []
     for combo in itertools.product(*(grid[k] for k in keys)):
         params = dict(zip(keys, combo))
-        r = run_once(G, batches, budget, deadline_ms, params, mem_budget)
+        r = run_once(G, batches, budget, deadline_ms, params, mem_budget, inflight)
         row = {**params, **r}
         rows.append(row)
         print(f"[tune] {json.dumps(row)}", file=log, flush=True)
@@ -100,6 +122,8 @@ def main():
     ap.add_argument("--deadline-ms", type=int, default=2000)
     ap.add_argument("--mem-gb", type=float, default=32.0)
     ap.add_argument("--grid", default="", help="override grid axes, e.g. 'waves_per_cu=4,8'")
+    ap.add_argument("--inflight", type=int, default=4,
+                    help="batches in the engine at once (continuous batching)")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
     g = configs.graph(args.config)
@@ -109,8 +133,9 @@ def main():
           for q in configs.queries(args.config, core, count=args.batches * args.batch)]
     batches = [qs[i:i + args.batch] for i in range(0, len(qs), args.batch)]
     best, rows = tune(G, batches, args.budget, args.deadline_ms, grid=parse_grid(args.grid),
-                      mem_budget=int(args.mem_gb * (1 << 30)))
+                      mem_budget=int(args.mem_gb * (1 << 30)), inflight=args.inflight)
     out = {"config": args.config, "queries": len(qs), "batch": args.batch,
+           "inflight": args.inflight,
            "budget": args.budget, "deadline_ms": args.deadline_ms, "objective":
            "proven-optimal queries/s, then kl sum", "best": best, "sweep": rows}
     print(json.dumps({"best": best}), flush=True)

@@ -374,22 +374,37 @@ __global__ void __launch_bounds__(256) ms_scatter(u32 n, u32 G, const u64* __res
       nbytes += 12;
     }
     u64 any = ms_or(m);
+    // eight queries at a time: their distance loads are in flight together
+    // before the stores that need them
     while (any) {
-      const int b = __ffsll((unsigned long long)any) - 1;
-      any &= any - 1;
-      const u64 bal = __ballot((m >> b) & 1ull);
-      const u32 base = (u32)__shfl((int)gb, b), qv = (u32)__shfl((int)qb, b);
-      u32* const lorig = (u32*)ms_shfl64((u64)Qb.lorig, b);
-      u32* const ldist = (u32*)ms_shfl64((u64)Qb.ldist, b);
-      u32* const lcore = (u32*)ms_shfl64((u64)Qb.lcore, b);
-      if ((m >> b) & 1ull) {
-        const u32 id = base + (u32)__popcll(bal & below);
-        lorig[id] = (u32)v;
-        ldist[id] = dist16[v * MS_BITS + b];
-        lcore[id] = cv;
-        locid[v * MS_BITS + b] = id;
-        if ((u32)v == qv) qloc[b] = id;
-        nbytes += 4 + 2 + 12 + 4;
+      int bb[8];
+      u32 dv[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        bb[t] = any ? __ffsll((unsigned long long)any) - 1 : -1;
+        any &= any - 1;  // 0 stays 0
+      }
+#pragma unroll
+      for (int t = 0; t < 8; t++)
+        dv[t] = (bb[t] >= 0 && ((m >> bb[t]) & 1ull)) ? (u32)dist16[v * MS_BITS + bb[t]] : 0u;
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const int b = bb[t];
+        if (b < 0) break;
+        const u64 bal = __ballot((m >> b) & 1ull);
+        const u32 base = (u32)__shfl((int)gb, b), qv = (u32)__shfl((int)qb, b);
+        u32* const lorig = (u32*)ms_shfl64((u64)Qb.lorig, b);
+        u32* const ldist = (u32*)ms_shfl64((u64)Qb.ldist, b);
+        u32* const lcore = (u32*)ms_shfl64((u64)Qb.lcore, b);
+        if ((m >> b) & 1ull) {
+          const u32 id = base + (u32)__popcll(bal & below);
+          lorig[id] = (u32)v;
+          ldist[id] = dv[t];
+          lcore[id] = cv;
+          locid[v * MS_BITS + b] = id;
+          if ((u32)v == qv) qloc[b] = id;
+          nbytes += 4 + 2 + 12 + 4;
+        }
       }
     }
   }
@@ -398,7 +413,9 @@ __global__ void __launch_bounds__(256) ms_scatter(u32 n, u32 G, const u64* __res
 
 // Whole-graph queries (gmask): query b's distance array over every vertex of
 // the graph, SCS_DIST_NONE where b does not keep the vertex.  Lane = vertex,
-// so each query's 64 values of a group are one coalesced store.
+// so each query's 64 values of a group are one coalesced store.  Queries are
+// taken eight at a time: their eight distance loads are in flight together
+// before the eight stores.
 __global__ void __launch_bounds__(256) ms_gdist(u32 n, u32 G, const u64* __restrict__ vis,
                                                 u64 gmask, const u16* __restrict__ dist16,
                                                 const MsBatch* __restrict__ B,
@@ -415,13 +432,24 @@ __global__ void __launch_bounds__(256) ms_gdist(u32 n, u32 G, const u64* __restr
     if (in) nbytes += 8;
     u64 left = gmask;
     while (left) {
-      const int b = __ffsll((unsigned long long)left) - 1;
-      left &= left - 1;
-      u32* const d = (u32*)ms_shfl64(dp, b);
-      if (in) {
-        const bool kept = (m >> b) & 1ull;
-        d[v] = kept ? (u32)dist16[v * MS_BITS + b] : SCS_DIST_NONE;
-        nbytes += kept ? 6 : 4;
+      int bb[8];
+      u32 dv[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        bb[t] = left ? __ffsll((unsigned long long)left) - 1 : -1;
+        left &= left - 1;  // 0 stays 0
+      }
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const bool kept = bb[t] >= 0 && ((m >> bb[t]) & 1ull);
+        dv[t] = kept ? (u32)dist16[v * MS_BITS + bb[t]] : SCS_DIST_NONE;
+        if (in && bb[t] >= 0) nbytes += kept ? 6 : 4;
+      }
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        if (bb[t] < 0) break;
+        u32* const d = (u32*)ms_shfl64(dp, bb[t]);
+        if (in) d[v] = dv[t];
       }
     }
   }
