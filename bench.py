@@ -269,7 +269,8 @@ def main():
     ap.add_argument("--learned", type=int, default=0,
                     help="1 = the main leg itself uses the learned order")
     ap.add_argument("--model", default=None,
-                    help="branching model (default models/branching_<config>.npz when present)")
+                    help="branching model (default models/branching_<config>.npz when present, "
+                         "else the C1 model)")
     ap.add_argument("--prep-sample", type=int, default=256,
                     help="queries preprocessed alone (batched) for the preprocessing roofline")
     ap.add_argument("--single-sample", type=int, default=16,
@@ -325,7 +326,12 @@ def main():
     warm = [allq[i * B:(i + 1) * B] for i in range(args.warmup)]
     timed = [allq[(args.warmup + i) * B:(args.warmup + i + 1) * B] for i in range(args.steps)]
     timed_flat = [q for b in timed for q in b]
+    # a workload's own model when it has one; otherwise the model trained on
+    # the proven optima of C1 (C4 gives no proven training queries: none of
+    # 800 within its budget and deadline)
     mpath = args.model or os.path.join(ROOT, "models", f"branching_{args.config.lower()}.npz")
+    if not args.model and not os.path.exists(mpath):
+        mpath = os.path.join(ROOT, "models", "branching_c1.npz")
     model = scs.Model.load(mpath) if os.path.exists(mpath) else None
     eng_kw = dict(node_budget=args.budget, waves_per_cu=args.waves_per_cu,
                   limit_queries=args.limit_queries, time_slice_us=args.slice_us,

@@ -21,6 +21,8 @@
 
 Usage:
   python -m scs_amd.learn --config C1 --out models/branching_c1.npz
+  python -m scs_amd.learn --config C4 --per-seed 8 --deadline-ms 2000 \
+      --out models/branching_c4.npz   # stops: C4 proves none of its training queries
 """
 
 from __future__ import annotations
@@ -127,8 +129,11 @@ def main():
     ap.add_argument("--limit-queries", type=int, default=1024)
     ap.add_argument("--levels", type=int, default=1)
     ap.add_argument("--pair", type=int, default=1)
+    ap.add_argument("--deadline-ms", type=int, default=0,
+                    help="per-query deadline from submission (0 = none), in collection and evaluation")
     args = ap.parse_args()
-    ekw = dict(limit_queries=args.limit_queries, levels=args.levels, pair=args.pair)
+    ekw = dict(limit_queries=args.limit_queries, levels=args.levels, pair=args.pair,
+               deadline_ms=args.deadline_ms, mem_budget=32 << 30)
 
     print(f"[learn] building {args.config}", file=sys.stderr, flush=True)
     g = configs.graph(args.config)
@@ -141,6 +146,9 @@ def main():
     t_col = time.perf_counter() - t0
     print(f"[learn] {len(tq)} training queries ({n_proven} proven) -> {len(y)} rows, "
           f"{int(y.sum())} positive ({t_col:.1f} s)", file=sys.stderr, flush=True)
+    if len(y) == 0:
+        sys.exit(f"[learn] no training query of {args.config} was proven within the budget and "
+                 "deadline, so there are no labels from proven optima to train on")
     t0 = time.perf_counter()
     m, loss, pw = train(X, y, epochs=args.epochs)
     t_tr = time.perf_counter() - t0
@@ -155,7 +163,8 @@ def main():
                                                  ev["plain"]["status"], ev["learned"]["status"])
                if s1 != "CAPPED" and s2 != "CAPPED")
     rep = {"config": args.config, "train_queries": len(tq), "train_proven": n_proven,
-           "levels": args.levels, "pair": args.pair, "rows": int(len(y)),
+           "levels": args.levels, "pair": args.pair, "deadline_ms": args.deadline_ms,
+           "rows": int(len(y)),
            "positives": int(y.sum()), "loss": loss, "collect_s": t_col, "train_s": t_tr,
            "eval_queries": len(eq), "budget": args.budget,
            "plain": {k: ev["plain"][k] for k in ("seconds", "nodes", "proven")},
