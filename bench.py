@@ -15,13 +15,15 @@ submission) passes (status CAPPED: kl is then only a lower bound).  Warm-up
 steps take the stream's first batches and the timed steps the batches after
 them, so no query is timed twice.
 
-value = proven-optimal queries per second over the timed steps: CAPPED
-answers are excluded from the numerator and counted beside it, with the
-proven fraction.  Beside it the line reports, on the same timed queries:
-the learned branching order (batched), the plain order one query at a time,
-the CPU oracle (same rules, same budget, one core, on a bounded sample) and
-whether every optimum both proved is equal; the preprocessing kernels alone;
-and the roofline of the search kernel.
+value = proven-optimal queries per second over the timed steps, in the
+serving form: batched, with the learned branching order (the workload's
+model, else the one trained on C1's proven optima).  CAPPED answers are
+excluded from the numerator and counted beside it, with the proven
+fraction.  Beside it the line reports, on the same timed queries: the plain
+order (batched), the plain order one query at a time, the CPU oracle (same
+rules, same budget, one core, on a bounded sample) and whether every
+optimum two forms proved is equal; the preprocessing kernels alone; and the
+roofline of the search kernel.
 
 Prints one JSON line (rank 0).
 """
@@ -266,8 +268,9 @@ def main():
     ap.add_argument("--admit-per-step", type=int, default=256)
     ap.add_argument("--levels", type=int, default=1, help="descending level floors")
     ap.add_argument("--pair", type=int, default=1, help="common-neighbour pair rule")
-    ap.add_argument("--learned", type=int, default=0,
-                    help="1 = the main leg itself uses the learned order")
+    ap.add_argument("--learned", type=int, default=1,
+                    help="1 = the main leg uses the learned order (the serving form; the plain "
+                         "order runs beside it), 0 = the main leg is the plain order")
     ap.add_argument("--model", default=None,
                     help="branching model (default models/branching_<config>.npz when present, "
                          "else the C1 model)")
@@ -348,10 +351,11 @@ def main():
             out["prep_standalone_local"] = prep_leg(scs, G, core, sample, whole=0)
         print(json.dumps(out), flush=True)
         return
+    # the main leg runs the learned order (the serving form) when a model is
+    # present; the other order runs beside it on the same queries
+    learned = bool(args.learned) and model is not None
     E = scs.Engine(G, **eng_kw)
-    if args.learned:
-        if model is None:
-            sys.exit("--learned needs a model")
+    if learned:
         E.set_model(model, True)
     # code objects load outside the timed region
     E.run([allq[0]])
@@ -394,7 +398,7 @@ def main():
         "config": {"workload": f"{args.config}: {gdesc}, {args.steps} batches of {B} queries "
                                f"from the stream",
                    "node_budget": args.budget, "deadline_ms": args.deadline_ms,
-                   "learned": bool(args.learned),
+                   "learned": learned,
                    "levels": args.levels, "pair": args.pair, "waves": s1["waves"],
                    "limit_queries": args.limit_queries, "inflight": args.inflight},
         "proven": main_sum["proven"],
@@ -433,8 +437,9 @@ def main():
                                               ("batch", B), ("levels", args.levels),
                                               ("pair", args.pair),
                                               ("deadline_ms", args.deadline_ms),
-                                              ("inflight", args.inflight)))
-        if same and not args.learned:
+                                              ("inflight", args.inflight),
+                                              ("learned", int(learned))))
+        if same:
             line["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
             line["roofline"]["traffic_source"] = os.path.relpath(prof, ROOT)
     by_tag = {r.tag: r for r in res}
@@ -444,11 +449,13 @@ def main():
                 if r.status in PROVEN and r.tag in by_tag and by_tag[r.tag].status in PROVEN]
         return len(both), sum(a.kl == b.kl for a, b in both)
 
-    # the same timed queries in the learned order (batched)
-    if model is not None and not args.learned:
-        print("[bench] learned leg", file=sys.stderr, flush=True)
+    # the same timed queries in the other order (batched)
+    if model is not None:
+        other = "plain" if learned else "learned"
+        print(f"[bench] {other} leg", file=sys.stderr, flush=True)
         EL = scs.Engine(G, **eng_kw)
-        EL.set_model(model, True)
+        if not learned:
+            EL.set_model(model, True)
         EL.run([allq[0]])
         t0 = time.perf_counter()
         lres = run_batches(EL, timed, progress=20.0, inflight=args.inflight)
@@ -456,10 +463,14 @@ def main():
         EL.close()
         ls = summary(lres, lw)
         nb, ne = agree(lres)
-        ls.update({"model": os.path.relpath(mpath, ROOT), "both_proven": nb,
-                   "optimum_equal": ne == nb})
-        line["learned_batched"] = ls
-        line["learned_vs_plain_batched"] = round(ls["value"] / max(1e-9, main_sum["value"]), 3)
+        ls.update({"both_proven": nb, "optimum_equal": ne == nb})
+        if not learned:
+            ls["model"] = os.path.relpath(mpath, ROOT)
+        line[f"{other}_batched"] = ls
+        lv, pv = (main_sum["value"], ls["value"]) if learned else (ls["value"], main_sum["value"])
+        line["learned_vs_plain_batched"] = round(lv / max(1e-9, pv), 3)
+    if learned:
+        line["config"]["model"] = os.path.relpath(mpath, ROOT)
     # the plain order one query at a time (one resident query), on a sample
     # of the timed queries, against the batched rate
     if args.single_sample > 0:
@@ -478,9 +489,9 @@ def main():
         line["plain_one_at_a_time"] = ss
         if ss["value"] > 0:
             line["batched_vs_one_at_a_time"] = round(main_sum["value"] / ss["value"], 2)
-            if "learned_batched" in line:
-                line["learned_batched_vs_one_at_a_time"] = round(
-                    line["learned_batched"]["value"] / ss["value"], 2)
+            for k in ("learned_batched", "plain_batched"):
+                if k in line:
+                    line[f"{k}_vs_one_at_a_time"] = round(line[k]["value"] / ss["value"], 2)
 
     # batched preprocessing alone (no search kernel sharing the GPU): the
     # roofline of the preprocessing kernels
