@@ -117,7 +117,9 @@ typedef struct scs_params {
                               holds at least half the graph (default), 2 = the resident
                               CSR whenever the search state fits; same optimum, node
                               counts and witness in every mode */
-  uint32_t reserved[1];
+  uint32_t gpu_heuristic;  /* 1 = admission computes CSSC_heu (inc/heuristic.h:321-348) on
+                              the GPU, one wave per query, instead of on host threads;
+                              same kl and H (needs fewer than 2^32 adjacency entries) */
 } scs_params;
 
 typedef struct scs_result {
@@ -216,6 +218,15 @@ int scs_engine_stats(const scs_engine* e, scs_stats* s);
 int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_t n, uint32_t l,
                       uint32_t h, uint32_t q, uint32_t* kl, uint32_t* H, uint32_t cap,
                       uint32_t* hsize);
+
+/* The same greedy lower bound (inc/heuristic.h:8-348) for `count` queries on
+ * the GPU, one wave per query, on a resident graph: kl[i], hsize[i] (the size
+ * of H, also when it exceeds cap) and H row i (up to cap ids).  Equal to
+ * scs_heuristic_csr value for value.  SCS_EBIG when the graph has 2^32
+ * adjacency entries or more. */
+int scs_heuristic_gpu(const scs_graph* g, uint32_t count, const uint32_t* l,
+                      const uint32_t* h, const uint32_t* q, uint32_t* kl, uint32_t* hsize,
+                      uint32_t* H, uint32_t cap);
 
 /* Runs the GPU per-query preprocessing (bounded BFS, compaction, local CSR)
  * for (q, kl0, ubD) and copies the result out: K kept vertices (original

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libscs.so")
 SERVER = os.path.join(HERE, "server", "scs_server")
 CLIENT = os.path.join(HERE, "server", "scs_client")
 
-HIP_SOURCES = ["scs_graph.hip", "scs_prep.hip", "scs_engine.hip", "scs_mlp.hip"]
+HIP_SOURCES = ["scs_graph.hip", "scs_prep.hip", "scs_engine.hip", "scs_mlp.hip", "scs_heu.hip"]
 CPP_SOURCES = ["scs_capi.cpp", "scs_heuristic.cpp"]
 ARCH = os.environ.get("SCS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -65,6 +65,10 @@ hipError_t scs_launch_expire(QuerySlot* slots, const u32* idx, u32 n, hipStream_
 hipError_t scs_launch_admit(const EngineArgs* A, const u32* slots, u32 n, u32* failed,
                             hipStream_t s);
 hipError_t scs_launch_pool_init(u64* seq, u32 n, hipStream_t s);
+HeuLayout scs_heu_layout(u32 n, u32 hcap);
+hipError_t scs_launch_heu(const u32* off, const u32* adj, u32 n, u32 dmax, u32 nq, const u32* ql,
+                          const u32* qh, const u32* qq, u8* ws, HeuLayout lay, u32 hcap, u32* kl,
+                          u32* hsize, u32* H, hipStream_t s);
 }
 
 #define HIPCHK(x)                                                                   \
@@ -310,6 +314,90 @@ extern "C" void scs_graph_destroy(scs_graph* g) {
 }
 
 // ---------------------------------------------------------------------------
+// GPU greedy heuristic: up to `waves` queries per launch, one wave each, on
+// the graph's 32-bit CSR; the workspace's n-sized arrays start zeroed and
+// every wave leaves its own zeroed again.
+// ---------------------------------------------------------------------------
+struct HeuGpu {
+  u8* ws = nullptr;
+  HeuLayout lay{};
+  u32 waves = 0;
+  u32* d_in = nullptr;   // l, h, q (3 * waves)
+  u32* d_out = nullptr;  // kl, hsize (2 * waves)
+  u32* d_H = nullptr;    // waves * SCS_MAX_H
+  std::vector<u32> h_out, h_H;
+
+  int init(u32 n, u32 w) {
+    waves = w;
+    lay = scs_heu_layout(n, SCS_MAX_H);
+    HIPCHK(hipMalloc(&ws, lay.stride * waves));
+    HIPCHK(hipMemset(ws, 0, lay.stride * waves));
+    HIPCHK(hipMalloc(&d_in, sizeof(u32) * 3 * waves));
+    HIPCHK(hipMalloc(&d_out, sizeof(u32) * 2 * waves));
+    HIPCHK(hipMalloc(&d_H, sizeof(u32) * (u64)SCS_MAX_H * waves));
+    h_out.resize(2 * waves);
+    h_H.resize((size_t)SCS_MAX_H * waves);
+    return SCS_OK;
+  }
+  void release() {
+    for (void* p : {(void*)ws, (void*)d_in, (void*)d_out, (void*)d_H})
+      if (p) (void)hipFree(p);
+    ws = nullptr;
+    d_in = d_out = d_H = nullptr;
+  }
+  // queries i0..i0+cnt (cnt <= waves): kl, and H as vectors
+  int run(const scs_graph* g, u32 cnt, const u32* l, const u32* h, const u32* q, u32* kl,
+          std::vector<u32>* H, hipStream_t s) {
+    std::vector<u32> in(3 * (size_t)cnt);
+    for (u32 i = 0; i < cnt; i++) {
+      in[i] = l[i];
+      in[cnt + i] = h[i];
+      in[2 * cnt + i] = q[i];
+    }
+    HIPCHK(hipMemcpyAsync(d_in, in.data(), sizeof(u32) * 3 * cnt, hipMemcpyHostToDevice, s));
+    HIPCHK(scs_launch_heu(g->d_off32, (const u32*)g->d_adj, g->hg.n, g->hg.dmax, cnt, d_in,
+                          d_in + cnt, d_in + 2 * cnt, ws, lay, SCS_MAX_H, d_out, d_out + cnt, d_H,
+                          s));
+    HIPCHK(hipMemcpyAsync(h_out.data(), d_out, sizeof(u32) * 2 * cnt, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h_H.data(), d_H, sizeof(u32) * (u64)SCS_MAX_H * cnt,
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (u32 i = 0; i < cnt; i++) {
+      kl[i] = h_out[i];
+      const u32 sz = std::min<u32>(h_out[cnt + i], SCS_MAX_H);
+      H[i].assign(h_H.begin() + (size_t)i * SCS_MAX_H, h_H.begin() + (size_t)i * SCS_MAX_H + sz);
+    }
+    return SCS_OK;
+  }
+};
+
+extern "C" int scs_heuristic_gpu(const scs_graph* g, uint32_t count, const uint32_t* l,
+                                 const uint32_t* h, const uint32_t* q, uint32_t* kl,
+                                 uint32_t* hsize, uint32_t* H, uint32_t cap) {
+  if (!g || !l || !h || !q || !kl || !hsize || (cap && !H)) return SCS_EINVAL;
+  if (!g->d_off32) return SCS_EBIG;
+  for (u32 i = 0; i < count; i++)
+    if (q[i] >= g->hg.n || h[i] == 0 || h[i] > SCS_MAX_H || l[i] > h[i]) return SCS_EINVAL;
+  if (count == 0) return SCS_OK;
+  HeuGpu hg;
+  int rc = hg.init(g->hg.n, std::min<u32>(count, 64));
+  hipStream_t s = nullptr;
+  if (rc == SCS_OK && hipStreamCreate(&s) != hipSuccess) rc = SCS_EHIP;
+  std::vector<std::vector<u32>> HH(hg.waves);
+  for (u32 i0 = 0; i0 < count && rc == SCS_OK; i0 += hg.waves) {
+    const u32 cnt = std::min<u32>(hg.waves, count - i0);
+    rc = hg.run(g, cnt, l + i0, h + i0, q + i0, kl + i0, HH.data(), s);
+    for (u32 i = 0; i < cnt && rc == SCS_OK; i++) {
+      hsize[i0 + i] = hg.h_out[cnt + i];
+      for (u32 k = 0; k < HH[i].size() && k < cap; k++) H[(u64)(i0 + i) * cap + k] = HH[i][k];
+    }
+  }
+  if (s) (void)hipStreamDestroy(s);
+  hg.release();
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
 // Engine
 // ---------------------------------------------------------------------------
 struct PendingQuery {
@@ -387,6 +475,7 @@ struct scs_engine {
   std::vector<QuerySlot> h_slots;
   std::deque<scs_result> results;
   std::vector<scs::HeuWork> heu_work;
+  HeuGpu heu_gpu;  // used when p.gpu_heuristic (waves == 0 otherwise)
   u32 active = 0;
   scs_stats st{};
 };
@@ -609,6 +698,9 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   E->slots.resize(nq);
   E->h_slots.resize(nq);
   E->heu_work.resize(p.heuristic_threads);
+  if (p.gpu_heuristic && g->d_off32) {
+    if (E->heu_gpu.init(g->hg.n, 64) != SCS_OK) return fail(SCS_ENOMEM);
+  }
   E->st.waves = E->nwaves;
   E->st.kcap = E->kcap;
   if (hipMalloc(&E->d_model, sizeof(MlpDev)) != hipSuccess) return fail(SCS_ENOMEM);
@@ -675,6 +767,7 @@ extern "C" void scs_engine_destroy(scs_engine* E) {
   if (!E) return;
   if (E->s) (void)hipStreamSynchronize(E->s);
   if (E->ps) (void)hipStreamSynchronize(E->ps);
+  E->heu_gpu.release();
   void* ptrs[] = {E->d_slots, E->d_witness, E->d_wave, E->d_pool_seq, E->d_pool_data,
                   E->d_pool_pos, E->d_glob, E->d_elig, E->d_vis, E->d_fr, E->d_nf, E->d_dist16,
                   E->d_locid64, E->d_gcnt, E->d_units, E->d_hubs, E->d_hubcnt, E->d_csum,
@@ -1095,7 +1188,30 @@ extern "C" int scs_step(scs_engine* E) {
   // 2. heuristic lower bounds on host threads
   std::vector<u32> kl0(batch.size(), 0);
   std::vector<std::vector<u32>> H(batch.size());
-  if (!batch.empty()) {
+  if (!batch.empty() && E->heu_gpu.waves) {
+    // on the GPU, on the preprocessing stream, 64 queries per launch
+    double t0 = now_us();
+    std::vector<u32> idx, gl, gh, gq;
+    for (size_t i = 0; i < batch.size(); i++)
+      if (batch[i].heu && batch[i].l <= batch[i].h) {
+        idx.push_back((u32)i);
+        gl.push_back(batch[i].l);
+        gh.push_back(batch[i].h);
+        gq.push_back(batch[i].q);
+      }
+    std::vector<u32> gk(idx.size());
+    std::vector<std::vector<u32>> GH(E->heu_gpu.waves);
+    for (size_t i0 = 0; i0 < idx.size(); i0 += E->heu_gpu.waves) {
+      const u32 cnt = (u32)std::min<size_t>(E->heu_gpu.waves, idx.size() - i0);
+      int rc = E->heu_gpu.run(E->g, cnt, &gl[i0], &gh[i0], &gq[i0], &gk[i0], GH.data(), E->ps);
+      if (rc) return rc;
+      for (u32 i = 0; i < cnt; i++) {
+        kl0[idx[i0 + i]] = gk[i0 + i];
+        H[idx[i0 + i]].swap(GH[i]);
+      }
+    }
+    E->st.heu_us += now_us() - t0;
+  } else if (!batch.empty()) {
     double t0 = now_us();
     std::atomic<size_t> next{0};
     auto worker = [&](u32 t) {

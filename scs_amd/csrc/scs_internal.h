@@ -220,3 +220,15 @@ struct EngineArgs {
   u32 lds_words;       // dynamic LDS words per wave (set by the launcher)
   u32 no_bitmaps;      // 1 = no LDS bitmaps (status from device memory only)
 };
+
+// GPU greedy heuristic (scs_heu.hip): byte offsets of one wave's workspace
+// (every n-sized array is all-zero between queries) and its two counters.
+struct HeuLayout {
+  u64 sta, ins, cnt, deg, key, pos, list, rmv, th, bkey, bkd, bki, dbits, dlist, lds, stride;
+};
+struct HeuLds {
+  u32 m;       // candidates in the list
+  u32 ndirty;  // dirty blocks
+  u32 nH;
+  u32 pad;
+};

@@ -47,7 +47,7 @@ class Params(C.Structure):
                 ("model", C.c_void_p), ("heuristic_threads", C.c_uint32),
                 ("admit_per_step", C.c_uint32), ("no_lds_bitmaps", C.c_uint32),
                 ("levels", C.c_uint32), ("pair", C.c_uint32), ("deadline_ms", C.c_uint32),
-                ("whole_graph", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("whole_graph", C.c_uint32), ("gpu_heuristic", C.c_uint32)]
 
 
 class Result(C.Structure):
@@ -234,6 +234,25 @@ class QueryResult:
 
 def _commas(x: int) -> str:
     return f"{x:,}"
+
+
+def heuristic_gpu(graph: "Graph", queries, cap=MAX_H):
+    """CSSC_heu on the GPU for a list of (l, h, q): [(kl, H)] per query, H as
+    a uint32 array (cut at cap ids)."""
+    n = len(queries)
+    if n == 0:
+        return []
+    l = np.ascontiguousarray([x[0] for x in queries], np.uint32)
+    h = np.ascontiguousarray([x[1] for x in queries], np.uint32)
+    q = np.ascontiguousarray([x[2] for x in queries], np.uint32)
+    kl = np.zeros(n, np.uint32)
+    hs = np.zeros(n, np.uint32)
+    H = np.zeros(n * cap, np.uint32)
+    f = lib().scs_heuristic_gpu
+    u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+    f.argtypes = [C.c_void_p, C.c_uint32, u32p, u32p, u32p, u32p, u32p, u32p, C.c_uint32]
+    _check(f(graph._h, n, l, h, q, kl, hs, H, cap), "scs_heuristic_gpu")
+    return [(int(kl[i]), H[i * cap:i * cap + min(int(hs[i]), cap)].copy()) for i in range(n)]
 
 
 def model_init(seed=0):
