@@ -261,6 +261,8 @@ def main():
     ap.add_argument("--deadline-ms", type=int, default=2000,
                     help="per-query wall-clock bound from submission (CAPPED past it)")
     ap.add_argument("--heuristic-threads", type=int, default=12)
+    ap.add_argument("--gpu-heuristic", type=int, default=0,
+                    help="1 = admission computes the greedy heuristic on the GPU")
     ap.add_argument("--waves-per-cu", type=int, default=4)
     ap.add_argument("--limit-queries", type=int, default=256)
     ap.add_argument("--slice-us", type=int, default=50000)
@@ -340,6 +342,7 @@ def main():
                   limit_queries=args.limit_queries, time_slice_us=args.slice_us,
                   copy_limit=args.copy_limit, mem_budget=32 << 30,
                   heuristic_threads=args.heuristic_threads, admit_per_step=args.admit_per_step,
+                  gpu_heuristic=args.gpu_heuristic,
                   levels=args.levels, pair=args.pair, deadline_ms=args.deadline_ms)
     if args.prep_only:
         sample = allq[:args.prep_sample or len(allq)]
@@ -400,7 +403,8 @@ def main():
                    "node_budget": args.budget, "deadline_ms": args.deadline_ms,
                    "learned": learned,
                    "levels": args.levels, "pair": args.pair, "waves": s1["waves"],
-                   "limit_queries": args.limit_queries, "inflight": args.inflight},
+                   "limit_queries": args.limit_queries, "inflight": args.inflight,
+                   "gpu_heuristic": args.gpu_heuristic},
         "proven": main_sum["proven"],
         "queries": main_sum["queries"],
         "proven_optimal_fraction": main_sum["proven_optimal_fraction"],
