@@ -699,7 +699,12 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   E->h_slots.resize(nq);
   E->heu_work.resize(p.heuristic_threads);
   if (p.gpu_heuristic && g->d_off32) {
-    if (E->heu_gpu.init(g->hg.n, 64) != SCS_OK) return fail(SCS_ENOMEM);
+    // as many waves as an admission step takes queries, within 4 GiB of
+    // workspace (a launch lasts as long as its slowest query)
+    const u64 stride = scs_heu_layout(g->hg.n, SCS_MAX_H).stride;
+    const u32 w = (u32)std::max<u64>(16, std::min<u64>(std::max<u32>(p.admit_per_step, 64),
+                                                       (4ull << 30) / std::max<u64>(stride, 1)));
+    if (E->heu_gpu.init(g->hg.n, w) != SCS_OK) return fail(SCS_ENOMEM);
   }
   E->st.waves = E->nwaves;
   E->st.kcap = E->kcap;
@@ -1189,7 +1194,8 @@ extern "C" int scs_step(scs_engine* E) {
   std::vector<u32> kl0(batch.size(), 0);
   std::vector<std::vector<u32>> H(batch.size());
   if (!batch.empty() && E->heu_gpu.waves) {
-    // on the GPU, on the preprocessing stream, 64 queries per launch
+    // on the GPU, on the preprocessing stream, up to heu_gpu.waves queries
+    // per launch
     double t0 = now_us();
     std::vector<u32> idx, gl, gh, gq;
     for (size_t i = 0; i < batch.size(); i++)

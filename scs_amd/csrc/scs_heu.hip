@@ -19,10 +19,12 @@
 // so the doubles are bit-identical.  heu4 drops the reference's +1 on q when
 // q is popped alone (inc/heuristic.h:259), as the host version does.
 //
-// Candidates sit in an append-only list; each block of 64 list positions
-// keeps its best key, and a step recomputes only the blocks whose members
-// changed (all of them when most did), so a step costs the changed
-// vertices plus one pass over the block keys.
+// Candidates sit in an append-only list.  Each block of 64 list positions,
+// and each group of 64 blocks, keeps the largest key of its members; keys
+// only rise while a vertex is a candidate, so a rise is an atomic max on its
+// block and group, and only the block and group of the vertex chosen in a
+// step are recomputed.  A step costs the chosen vertex's adjacency plus one
+// pass over the group keys (n / 4096 of them).
 #include "scs_internal.h"
 
 #define HEU_SMALL 64  // heu3: neighbours of at most this degree are scored by one lane
@@ -68,33 +70,9 @@ __device__ __forceinline__ u64 h_max64(u64 x) {
   }
   return x;
 }
-__device__ __forceinline__ u64 h_min64(u64 x) {
-  for (int o = 32; o > 0; o >>= 1) {
-    const u64 y = h_shfl64(x, (int)(LANE ^ (u32)o));
-    x = y < x ? y : x;
-  }
-  return x;
-}
 __device__ __forceinline__ double h_shfld(double x, int src) {
   return __longlong_as_double((long long)h_shfl64((u64)__double_as_longlong(x), src));
 }
-// (key, id) pairs ordered as std::pair<double, uint32_t>: the larger key,
-// then the larger id; invalid entries carry id INV and key -1
-__device__ __forceinline__ bool pair_gt(double a, u32 ai, double b, u32 bi) {
-  return a > b || (a == b && ai > bi && ai != 0xffffffffu) || (bi == 0xffffffffu && ai != 0xffffffffu);
-}
-__device__ __forceinline__ void h_maxpair(double& k, u32& id) {
-  for (int o = 32; o > 0; o >>= 1) {
-    const double k2 = h_shfld(k, (int)(LANE ^ (u32)o));
-    const u32 i2 = (u32)__shfl((int)id, (int)(LANE ^ (u32)o));
-    if (pair_gt(k2, i2, k, id)) {
-      k = k2;
-      id = i2;
-    }
-  }
-}
-
-
 struct HW {
   const u32* off;
   const u32* adj;
@@ -108,102 +86,115 @@ struct HW {
   u32* list;
   u32* rmv;
   u32* th;
-  u64* bkey;
-  double* bkd;
-  u32* bki;
-  u32* dbits;
-  u32* dlist;
+  u64* bkey;  // per block of 64 list positions: the largest member key (0 = none)
+  u64* gkey;  // per group of 64 blocks: the largest block key
   HeuLds* L;
 };
 
-__device__ __forceinline__ void mark_dirty(HW& w, u32 b) {
-  const u32 bit = 1u << (b & 31);
-  if (!(atomicOr(&w.dbits[b >> 5], bit) & bit)) w.dlist[atomicAdd(&w.L->ndirty, 1u)] = b;
+// Candidate keys, larger is chosen first, 0 = not a candidate.
+//   heu2: (count of H-neighbours + 1, id)
+//   heu4: the complement of (degree in S, id), so the smallest pair is largest
+//   heu3: the bits of the (non-negative) double key, plus one; ties are broken
+//         by id when the chosen vertex is looked up
+__device__ __forceinline__ u64 kv2(const HW& w, u32 x) {
+  return w.sta[x] == 1 ? (((u64)(w.cnt[x] + 1) << 32) | x) : 0ull;
+}
+__device__ __forceinline__ u64 kv4(const HW& w, u32 x) {
+  return w.ins[x] ? ~(((u64)w.deg[x] << 32) | x) : 0ull;
+}
+__device__ __forceinline__ u64 kv3(const HW& w, u32 x) {
+  return w.sta[x] == 1 ? (u64)__double_as_longlong(w.key[x]) + 1ull : 0ull;
+}
+template <int MODE>
+__device__ __forceinline__ u64 kv(const HW& w, u32 x) {
+  return MODE == 2 ? kv2(w, x) : (MODE == 4 ? kv4(w, x) : kv3(w, x));
+}
+// block and group keys are raised by atomics at the L2; reads go past the L1
+__device__ __forceinline__ u64 hld64(u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void hst64(u64* p, u64 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// a member's key rose (or it joined): raise its block's and group's keys
+__device__ __forceinline__ void bump(HW& w, u32 p, u64 k) {
+  atomicMax((unsigned long long*)&w.bkey[p >> 6], (unsigned long long)k);
+  atomicMax((unsigned long long*)&w.gkey[p >> 12], (unsigned long long)k);
 }
 
-// Block keys.  MODE 2: max of (cnt+1, id) over frontier members; MODE 4: min
-// of (deg, id) over members of S; MODE 3: max of (key, id) over frontier
-// members.  A block whose members all left holds 0 / ~0 / (-1, INV).
-// Eight blocks per pass (blocks blk[0..cnt), or 0..cnt-1 when blk is null),
-// so eight chains of dependent loads are in flight together.
+// Exact keys of blocks b0 .. b0+cnt (eight per pass, so eight chains of
+// dependent loads are in flight), then of the groups they fall in.
 template <int MODE>
-__device__ __forceinline__ void block_keys(HW& w, const u32* blk, u32 cnt, u32 m) {
+__device__ void blocks_exact(HW& w, u32 b0, u32 cnt, u32 m) {
   constexpr int U = 8;
   for (u32 i0 = 0; i0 < cnt; i0 += U) {
-    u32 b[U], x[U];
-    bool in[U];
+    u64 k[U];
 #pragma unroll
     for (int t = 0; t < U; t++) {
-      const u32 i = i0 + t;
-      b[t] = i < cnt ? (blk ? blk[i] : i) : 0u;
-      const u32 p = b[t] * 64 + LANE;
-      in[t] = i < cnt && p < m;
-      x[t] = in[t] ? w.list[p] : 0u;
+      const u32 p = (b0 + i0 + t) * 64 + LANE;
+      const bool in = i0 + t < cnt && p < m;
+      k[t] = in ? kv<MODE>(w, w.list[p]) : 0ull;
     }
-    if (MODE == 2) {
-      u64 k[U];
 #pragma unroll
-      for (int t = 0; t < U; t++)
-        k[t] = (in[t] && w.sta[x[t]] == 1) ? (((u64)(w.cnt[x[t]] + 1) << 32) | x[t]) : 0ull;
-#pragma unroll
-      for (int t = 0; t < U; t++) {
-        const u64 r = h_max64(k[t]);
-        if (LANE == 0 && i0 + t < cnt) w.bkey[b[t]] = r;
-      }
-    } else if (MODE == 4) {
-      u64 k[U];
-#pragma unroll
-      for (int t = 0; t < U; t++)
-        k[t] = (in[t] && w.ins[x[t]]) ? (((u64)w.deg[x[t]] << 32) | x[t]) : ~0ull;
-#pragma unroll
-      for (int t = 0; t < U; t++) {
-        const u64 r = h_min64(k[t]);
-        if (LANE == 0 && i0 + t < cnt) w.bkey[b[t]] = r;
-      }
-    } else {
-      double k[U];
-      u32 id[U];
-#pragma unroll
-      for (int t = 0; t < U; t++) {
-        const bool f = in[t] && w.sta[x[t]] == 1;
-        k[t] = f ? w.key[x[t]] : -1.0;
-        id[t] = f ? x[t] : 0xffffffffu;
-      }
-#pragma unroll
-      for (int t = 0; t < U; t++) {
-        h_maxpair(k[t], id[t]);
-        if (LANE == 0 && i0 + t < cnt) {
-          w.bkd[b[t]] = k[t];
-          w.bki[b[t]] = id[t];
+    for (int t = 0; t < U; t++) {
+      const u64 r = h_max64(k[t]);
+      if (LANE == 0 && i0 + t < cnt) hst64(&w.bkey[b0 + i0 + t], r);
+    }
+  }
+  __threadfence();
+  hsync();
+  const u32 nblk = (m + 63) / 64;
+  for (u32 g = (b0 >> 6); g <= ((b0 + cnt - 1) >> 6); g++) {
+    const u32 b = g * 64 + LANE;
+    const u64 r = h_max64(b < nblk ? hld64(&w.bkey[b]) : 0ull);
+    if (LANE == 0) hst64(&w.gkey[g], r);
+  }
+  __threadfence();
+  hsync();
+}
+
+// The largest key over all candidates (0 when there is none).
+__device__ __forceinline__ u64 top_key(HW& w, u32 m) {
+  const u32 ngrp = ((m + 63) / 64 + 63) / 64;
+  u64 best = 0;
+  for (u32 g = LANE; g < ngrp; g += 64) best = max(best, hld64(&w.gkey[g]));
+  return h_max64(best);
+}
+
+// heu3: the largest id among the candidates whose key is M.
+__device__ u32 find3(HW& w, u32 m, u64 M) {
+  const u32 nblk = (m + 63) / 64, ngrp = (nblk + 63) / 64;
+  u64 best = 0;
+  for (u32 g0 = 0; g0 < ngrp; g0 += 64) {
+    u64 gm = __ballot(g0 + LANE < ngrp && hld64(&w.gkey[g0 + LANE]) == M);
+    while (gm) {
+      const u32 g = g0 + (u32)(__ffsll((unsigned long long)gm) - 1);
+      gm &= gm - 1;
+      const u32 b1 = g * 64 + LANE;
+      u64 bm = __ballot(b1 < nblk && hld64(&w.bkey[b1]) == M);
+      while (bm) {
+        const u32 b = g * 64 + (u32)(__ffsll((unsigned long long)bm) - 1);
+        bm &= bm - 1;
+        const u32 p = b * 64 + LANE;
+        u64 c = 0;
+        if (p < m) {
+          const u32 x = w.list[p];
+          if (kv3(w, x) == M) c = (u64)x + 1;
         }
+        best = max(best, h_max64(c));
       }
     }
   }
+  return best ? (u32)(best - 1) : 0xffffffffu;
 }
 
-// Recomputes the dirty blocks (every block when most are dirty) and clears
-// the dirty marks.
-template <int MODE>
-__device__ __forceinline__ void refresh(HW& w) {
-  hsync();
-  const u32 m = hld(&w.L->m), nblk = (m + 63) / 64, nd = hld(&w.L->ndirty);
-  if (nd == 0) return;
-  if (nd * 4 >= nblk)
-    block_keys<MODE>(w, nullptr, nblk, m);
-  else
-    block_keys<MODE>(w, w.dlist, nd, m);
-  for (u32 i = LANE; i < nd; i += 64) w.dbits[w.dlist[i] >> 5] = 0u;
-  hsync();
-  if (LANE == 0) w.L->ndirty = 0;
-  hsync();
-}
-
-// Appends v to the candidate list (one lane per call site).
-__device__ __forceinline__ void push_cand(HW& w, u32 v) {
+// Appends v to the candidate list (one lane per call site); its key is
+// raised by the caller.
+__device__ __forceinline__ u32 push_cand(HW& w, u32 v) {
   const u32 p = atomicAdd(&w.L->m, 1u);
   w.list[p] = v;
   w.pos[v] = p;
-  mark_dirty(w, p >> 6);
+  return p;
 }
 
 // min over H = th[0..nH) of deg
@@ -213,11 +204,11 @@ __device__ __forceinline__ u32 min_deg_h(HW& w, u32 nH) {
   return h_min(mn);
 }
 
+// Every array the run touched back to zero: the candidates' state and the
+// block and group keys.
 __device__ __forceinline__ void cleanup(HW& w) {
   hsync();
-  const u32 m = hld(&w.L->m), nd = hld(&w.L->ndirty);
-  // dirty marks left by the last step (never refreshed)
-  for (u32 i = LANE; i < nd; i += 64) w.dbits[w.dlist[i] >> 5] = 0u;
+  const u32 m = hld(&w.L->m);
   for (u32 i = LANE; i < m; i += 64) {
     const u32 x = w.list[i];
     w.sta[x] = 0;
@@ -226,35 +217,34 @@ __device__ __forceinline__ void cleanup(HW& w) {
     w.deg[x] = 0;
     w.key[x] = 0.0;
   }
+  const u32 nblk = (m + 63) / 64, ngrp = (nblk + 63) / 64;
+  for (u32 b = LANE; b < nblk; b += 64) hst64(&w.bkey[b], 0ull);
+  for (u32 g = LANE; g < ngrp; g += 64) hst64(&w.gkey[g], 0ull);
+  __threadfence();
   hsync();
-  if (LANE == 0) {
-    w.L->m = 0;
-    w.L->ndirty = 0;
-  }
+  if (LANE == 0) __hip_atomic_store(&w.L->m, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __threadfence();
   hsync();
 }
 
 // heu2 (inc/heuristic.h:8-120): H grows from q by the frontier vertex with
 // the most H-neighbours (ties: larger id); the best prefix of size in
 // [N1, N2] by minimum degree (ties: the later one) is kept.
-__device__ __forceinline__ void heu2(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u32 hcap, u32& osize, u32& okl) {
+__device__ __forceinline__ void heu2(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u32 hcap,
+                                     u32& osize, u32& okl) {
   if (LANE == 0) {
     w.sta[q] = 1;
-    push_cand(w, q);
+    bump(w, push_cand(w, q), kv2(w, q));
   }
-  refresh<2>(w);
+  hsync();
   u32 nH = 0, tkl = 0, tsize = 0;
   while (true) {
-    const u32 nblk = (hld(&w.L->m) + 63) / 64;
-    u64 best = 0;
-    for (u32 b = LANE; b < nblk; b += 64) best = max(best, w.bkey[b]);
-    best = h_max64(best);
-    if (best == 0) break;
-    const u32 v = (u32)best;
+    const u64 M = top_key(w, hld(&w.L->m));
+    if (M == 0) break;
+    const u32 v = (u32)M;
     if (LANE == 0) {
       w.th[nH] = v;
       w.sta[v] = 2;
-      mark_dirty(w, w.pos[v] >> 6);
     }
     nH++;
     hsync();
@@ -262,21 +252,25 @@ __device__ __forceinline__ void heu2(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
     const u32 b0 = w.off[v], e0 = w.off[v + 1];
     for (u32 j = b0 + LANE; j < e0; j += 64) {
       const u32 nb = w.adj[j];
-      w.cnt[nb] += 1;
+      const u32 c = w.cnt[nb] + 1;
+      w.cnt[nb] = c;
       const u8 s = w.sta[nb];
+      const u64 k = ((u64)(c + 1) << 32) | nb;
       if (s == 2) {
         w.deg[nb] += 1;
         dv++;
       } else if (s == 0) {
         w.sta[nb] = 1;
-        push_cand(w, nb);
+        bump(w, push_cand(w, nb), k);
       } else {
-        mark_dirty(w, w.pos[nb] >> 6);
+        bump(w, w.pos[nb], k);
       }
     }
     dv = h_sum(dv);
     if (LANE == 0) w.deg[v] += dv;
     hsync();
+    // v left the frontier: its block and group are recomputed exactly
+    blocks_exact<2>(w, w.pos[v] >> 6, 1, hld(&w.L->m));
     if (nH >= N1) {
       const u32 mn = min_deg_h(w, nH);
       if (mn >= tkl) {
@@ -285,7 +279,6 @@ __device__ __forceinline__ void heu2(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
       }
     }
     if (nH == N2) break;
-    refresh<2>(w);
   }
   for (u32 i = LANE; i < tsize && i < hcap; i += 64) outH[i] = w.th[i];
   osize = tsize;
@@ -295,13 +288,16 @@ __device__ __forceinline__ void heu2(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
 
 // Score of a frontier vertex nb whose degree is at most HEU_SMALL, by one
 // lane: 1/deg(w) over nb's H-neighbours w with deg(w) != 0 in ascending id
-// order, then deg(nb)/dmax.
-__device__ __forceinline__ double score_lane(const HW& w, u32 nb) {
+// order, then deg(nb)/dmax.  nb has `target` H-neighbours (the count kept in
+// cnt during heu3), so the walk stops at the last of them.
+__device__ __forceinline__ double score_lane(const HW& w, u32 nb, u32 target) {
   const u32 b = w.off[nb], e = w.off[nb + 1];
   double sc = 0.0;
-  for (u32 k = b; k < e; k++) {
+  u32 found = 0;
+  for (u32 k = b; k < e && found < target; k++) {
     const u32 x = w.adj[k];
     if (w.sta[x] == 2) {
+      found++;
       const u32 d = w.deg[x];
       if (d != 0) sc += 1.0 / (double)d;
     }
@@ -310,17 +306,20 @@ __device__ __forceinline__ double score_lane(const HW& w, u32 nb) {
 }
 
 // The same sum by the whole wave for a vertex of large degree: chunks of 64
-// entries, the chunk's terms added one after another in lane order.
-__device__ __forceinline__ double score_wave(const HW& w, u32 nb) {
+// entries, the chunk's terms added one after another in lane order, until
+// the last H-neighbour has been met.
+__device__ __forceinline__ double score_wave(const HW& w, u32 nb, u32 target) {
   const u32 b = w.off[nb], e = w.off[nb + 1];
   double sc = 0.0;
-  for (u32 c = b; c < e; c += 64) {
+  u32 found = 0;
+  for (u32 c = b; c < e && found < target; c += 64) {
     const u32 k = c + LANE;
     double t = 0.0;
-    bool on = false;
+    bool inh = false, on = false;
     if (k < e) {
       const u32 x = w.adj[k];
       if (w.sta[x] == 2) {
+        inh = true;
         const u32 d = w.deg[x];
         if (d != 0) {
           t = 1.0 / (double)d;
@@ -328,6 +327,7 @@ __device__ __forceinline__ double score_wave(const HW& w, u32 nb) {
         }
       }
     }
+    found += (u32)__popcll(__ballot(inh));
     u64 msk = __ballot(on);
     while (msk) {
       const int i = __ffsll((unsigned long long)msk) - 1;
@@ -343,42 +343,33 @@ __device__ __forceinline__ void set_key3(HW& w, u32 nb, double sc) {
   if (s == 0) {
     w.key[nb] = sc;
     w.sta[nb] = 1;
-    push_cand(w, nb);
-  } else {
-    if (sc > w.key[nb]) w.key[nb] = sc;
-    mark_dirty(w, w.pos[nb] >> 6);
+    bump(w, push_cand(w, nb), (u64)__double_as_longlong(sc) + 1ull);
+  } else if (sc > w.key[nb]) {
+    w.key[nb] = sc;
+    bump(w, w.pos[nb], (u64)__double_as_longlong(sc) + 1ull);
   }
 }
 
 // heu3 (inc/heuristic.h:125-239): H grows from q by the frontier vertex of
 // largest connection score (the largest score pushed for it).
-__device__ __forceinline__ void heu3(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u32 hcap, u32& osize, u32& okl) {
+__device__ __forceinline__ void heu3(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u32 hcap,
+                                     u32& osize, u32& okl) {
   if (LANE == 0) {
     w.sta[q] = 1;
     w.key[q] = 0.0;
-    push_cand(w, q);
+    bump(w, push_cand(w, q), kv3(w, q));
   }
-  refresh<3>(w);
+  hsync();
   u32 nH = 0, tkl = 0, tsize = 0;
   while (true) {
-    const u32 nblk = (hld(&w.L->m) + 63) / 64;
-    double bk = -1.0;
-    u32 bi = 0xffffffffu;
-    for (u32 b = LANE; b < nblk; b += 64) {
-      const double k = w.bkd[b];
-      const u32 i = w.bki[b];
-      if (pair_gt(k, i, bk, bi)) {
-        bk = k;
-        bi = i;
-      }
-    }
-    h_maxpair(bk, bi);
-    if (bi == 0xffffffffu) break;
-    const u32 v = bi;
+    const u32 m = hld(&w.L->m);
+    const u64 M = top_key(w, m);
+    if (M == 0) break;
+    const u32 v = find3(w, m, M);
+    if (v == 0xffffffffu) break;  // keys and members disagree: stop (never seen)
     if (LANE == 0) {
       w.th[nH] = v;
       w.sta[v] = 2;
-      mark_dirty(w, w.pos[v] >> 6);
     }
     nH++;
     hsync();
@@ -386,6 +377,7 @@ __device__ __forceinline__ void heu3(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
     u32 dv = 0;
     for (u32 j = b0 + LANE; j < e0; j += 64) {
       const u32 nb = w.adj[j];
+      w.cnt[nb] += 1;  // heu3 keeps each vertex's number of H-neighbours in cnt
       if (w.sta[nb] == 2) {
         w.deg[nb] += 1;
         dv++;
@@ -394,34 +386,42 @@ __device__ __forceinline__ void heu3(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
     dv = h_sum(dv);
     if (LANE == 0) w.deg[v] += dv;
     hsync();
-    // scores of v's neighbours outside H, against the degrees after v joined
+    // scores of v's neighbours outside H, against the degrees after v joined;
+    // a vertex whose only H-neighbour is v needs no walk: its sum is v's term
+    const u32 degv = w.deg[v];
+    const double termv = degv != 0 ? 1.0 / (double)degv : 0.0;
     for (u32 c = b0; c < e0; c += 64) {
       const u32 j = c + LANE;
-      u32 nb = 0;
-      bool small = false, big = false;
+      u32 nb = 0, target = 0;
+      bool one = false, small = false, big = false;
       if (j < e0) {
         nb = w.adj[j];
         if (w.sta[nb] != 2) {
+          target = w.cnt[nb];
           const u32 dg = w.off[nb + 1] - w.off[nb];
-          small = dg <= HEU_SMALL;
-          big = !small;
+          one = target == 1;
+          small = !one && dg <= HEU_SMALL;
+          big = !one && !small;
         }
       }
-      double sc = 0.0;
-      if (small) sc = score_lane(w, nb);
-      // keys are written after every score of the chunk is known: a score
-      // reads only H's degrees, which no key update changes
-      if (small) set_key3(w, nb, sc);
+      // a score reads only H's degrees, which no key update changes
+      if (one) {
+        const u32 dg = w.off[nb + 1] - w.off[nb];
+        set_key3(w, nb, (0.0 + termv) + (double)dg / (double)w.dmax);
+      }
+      if (small) set_key3(w, nb, score_lane(w, nb, target));
       u64 bm = __ballot(big);
       while (bm) {
         const int src = __ffsll((unsigned long long)bm) - 1;
         bm &= bm - 1;
         const u32 x = (u32)__shfl((int)nb, src);
-        const double s2 = score_wave(w, x);
+        const u32 tg = (u32)__shfl((int)target, src);
+        const double s2 = score_wave(w, x, tg);
         if (LANE == 0) set_key3(w, x, s2);
       }
     }
     hsync();
+    blocks_exact<3>(w, w.pos[v] >> 6, 1, hld(&w.L->m));
     if (nH >= N1) {
       const u32 mn = min_deg_h(w, nH);
       if (mn >= tkl) {
@@ -430,7 +430,6 @@ __device__ __forceinline__ void heu3(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
       }
     }
     if (nH == N2) break;
-    refresh<3>(w);
   }
   for (u32 i = LANE; i < tsize && i < hcap; i += 64) outH[i] = w.th[i];
   osize = tsize;
@@ -451,7 +450,7 @@ __device__ __forceinline__ void heu4(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
     w.list[0] = q;
     w.pos[q] = 0;
     w.ins[q] = 1;
-    w.L->m = m;
+    __hip_atomic_store(&w.L->m, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   for (u32 j = qb + LANE; j < qe; j += 64) {
     const u32 x = w.adj[j];
@@ -488,15 +487,12 @@ __device__ __forceinline__ void heu4(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
     }
   }
   hsync();
-  const u32 nblk = (m + 63) / 64;
-  block_keys<4>(w, nullptr, nblk, m);
-  hsync();
+  blocks_exact<4>(w, 0, (m + 63) / 64, m);
   u32 nr = 0, rmv_idx = 0, mindeg = 0;
   while (true) {
-    u64 best = ~0ull;
-    for (u32 b = LANE; b < nblk; b += 64) best = min(best, w.bkey[b]);
-    best = h_min64(best);
-    if (best == ~0ull) break;
+    const u64 M = top_key(w, m);
+    if (M == 0) break;
+    const u64 best = ~M;
     const u32 v = (u32)best;
     u32 d = (u32)(best >> 32);
     const u32 remain = m - nr;
@@ -509,7 +505,6 @@ __device__ __forceinline__ void heu4(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
     if (LANE == 0) {
       w.ins[v] = 0;
       w.rmv[nr] = v;
-      mark_dirty(w, w.pos[v] >> 6);
     }
     nr++;
     hsync();
@@ -517,11 +512,13 @@ __device__ __forceinline__ void heu4(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
     for (u32 j = b0 + LANE; j < e0; j += 64) {
       const u32 x = w.adj[j];
       if (w.ins[x]) {
-        w.deg[x] -= 1;
-        mark_dirty(w, w.pos[x] >> 6);
+        const u32 d2 = w.deg[x] - 1;
+        w.deg[x] = d2;
+        bump(w, w.pos[x], ~(((u64)d2 << 32) | x));
       }
     }
-    refresh<4>(w);
+    hsync();
+    blocks_exact<4>(w, w.pos[v] >> 6, 1, m);
   }
   // H = S without the first rmv_idx peeled vertices, in S order
   for (u32 i = LANE; i < m; i += 64) w.ins[w.list[i]] = 1;
@@ -565,15 +562,10 @@ __global__ void __launch_bounds__(64) __attribute__((flatten)) scs_heu_kernel(He
   w.rmv = (u32*)(base + A.lay.rmv);
   w.th = (u32*)(base + A.lay.th);
   w.bkey = (u64*)(base + A.lay.bkey);
-  w.bkd = (double*)(base + A.lay.bkd);
-  w.bki = (u32*)(base + A.lay.bki);
-  w.dbits = (u32*)(base + A.lay.dbits);
-  w.dlist = (u32*)(base + A.lay.dlist);
+  w.gkey = (u64*)(base + A.lay.gkey);
   w.L = (HeuLds*)(base + A.lay.lds);
-  if (LANE == 0) {
-    w.L->m = 0;
-    w.L->ndirty = 0;
-  }
+  if (LANE == 0) __hip_atomic_store(&w.L->m, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __threadfence();
   hsync();
   const u32 l = A.ql[qi], h = A.qh[qi], q = A.qq[qi];
   u32* out = A.H + (u64)qi * A.hcap;
@@ -625,10 +617,7 @@ extern "C" HeuLayout scs_heu_layout(u32 n, u32 hcap) {
   L.rmv = o; o = heu_al(o + 4ull * (n + 64));
   L.th = o; o = heu_al(o + 4ull * (4ull * hcap + 64));
   L.bkey = o; o = heu_al(o + 8 * nblk);
-  L.bkd = o; o = heu_al(o + 8 * nblk);
-  L.bki = o; o = heu_al(o + 4 * nblk);
-  L.dbits = o; o = heu_al(o + 4 * (nblk / 32 + 1));
-  L.dlist = o; o = heu_al(o + 4 * nblk);
+  L.gkey = o; o = heu_al(o + 8 * (nblk / 64 + 2));
   L.lds = o; o = heu_al(o + sizeof(HeuLds));
   L.stride = o;
   return L;

@@ -224,11 +224,9 @@ struct EngineArgs {
 // GPU greedy heuristic (scs_heu.hip): byte offsets of one wave's workspace
 // (every n-sized array is all-zero between queries) and its two counters.
 struct HeuLayout {
-  u64 sta, ins, cnt, deg, key, pos, list, rmv, th, bkey, bkd, bki, dbits, dlist, lds, stride;
+  u64 sta, ins, cnt, deg, key, pos, list, rmv, th, bkey, gkey, lds, stride;
 };
 struct HeuLds {
-  u32 m;       // candidates in the list
-  u32 ndirty;  // dirty blocks
-  u32 nH;
-  u32 pad;
+  u32 m;  // candidates in the list
+  u32 pad[3];
 };
