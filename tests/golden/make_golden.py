@@ -12,7 +12,8 @@ same optimum, far fewer nodes), all 100 queries in parallel processes, each
 with node cap C1_CAP; a query the cap stops stays in the file as CAPPED with
 its lower bound.  A second pass (--retry-cap N) re-solves only those with a
 larger cap.  Run: python tests/golden/make_golden.py [--jobs N], then
-python tests/golden/make_golden.py --retry-cap 3000000
+python tests/golden/make_golden.py --retry-cap 1500000 (the file is rewritten after every
+answer, so a pass cut short keeps what it proved)
 """
 
 import json
