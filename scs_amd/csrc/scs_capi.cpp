@@ -68,7 +68,7 @@ hipError_t scs_launch_pool_init(u64* seq, u32 n, hipStream_t s);
 HeuLayout scs_heu_layout(u32 n, u32 hcap);
 hipError_t scs_launch_heu(const u32* off, const u32* adj, u32 n, u32 dmax, u32 nq, const u32* ql,
                           const u32* qh, const u32* qq, u8* ws, HeuLayout lay, u32 hcap, u32* kl,
-                          u32* hsize, u32* H, hipStream_t s);
+                          u32* hsize, u32* H, u32* kl3, u32* hs3, u32* H3, hipStream_t s);
 }
 
 #define HIPCHK(x)                                                                   \
@@ -314,9 +314,9 @@ extern "C" void scs_graph_destroy(scs_graph* g) {
 }
 
 // ---------------------------------------------------------------------------
-// GPU greedy heuristic: up to `waves` queries per launch, one wave each, on
-// the graph's 32-bit CSR; the workspace's n-sized arrays start zeroed and
-// every wave leaves its own zeroed again.
+// GPU greedy heuristic: up to `waves` queries per launch, three waves each
+// (one per construction), on the graph's 32-bit CSR; the workspace's n-sized
+// arrays start zeroed and every wave leaves its own zeroed again.
 // ---------------------------------------------------------------------------
 struct HeuGpu {
   u8* ws = nullptr;
@@ -325,25 +325,29 @@ struct HeuGpu {
   u32* d_in = nullptr;   // l, h, q (3 * waves)
   u32* d_out = nullptr;  // kl, hsize (2 * waves)
   u32* d_H = nullptr;    // waves * SCS_MAX_H
+  u32* d_out3 = nullptr;  // the three constructions: kl, hsize (2 * 3 * waves)
+  u32* d_H3 = nullptr;    // 3 * waves * SCS_MAX_H
   std::vector<u32> h_out, h_H;
 
   int init(u32 n, u32 w) {
     waves = w;
     lay = scs_heu_layout(n, SCS_MAX_H);
-    HIPCHK(hipMalloc(&ws, lay.stride * waves));
-    HIPCHK(hipMemset(ws, 0, lay.stride * waves));
+    HIPCHK(hipMalloc(&ws, lay.stride * 3 * waves));
+    HIPCHK(hipMemset(ws, 0, lay.stride * 3 * waves));
     HIPCHK(hipMalloc(&d_in, sizeof(u32) * 3 * waves));
     HIPCHK(hipMalloc(&d_out, sizeof(u32) * 2 * waves));
     HIPCHK(hipMalloc(&d_H, sizeof(u32) * (u64)SCS_MAX_H * waves));
+    HIPCHK(hipMalloc(&d_out3, sizeof(u32) * 6 * waves));
+    HIPCHK(hipMalloc(&d_H3, sizeof(u32) * (u64)SCS_MAX_H * 3 * waves));
     h_out.resize(2 * waves);
     h_H.resize((size_t)SCS_MAX_H * waves);
     return SCS_OK;
   }
   void release() {
-    for (void* p : {(void*)ws, (void*)d_in, (void*)d_out, (void*)d_H})
+    for (void* p : {(void*)ws, (void*)d_in, (void*)d_out, (void*)d_H, (void*)d_out3, (void*)d_H3})
       if (p) (void)hipFree(p);
     ws = nullptr;
-    d_in = d_out = d_H = nullptr;
+    d_in = d_out = d_H = d_out3 = d_H3 = nullptr;
   }
   // queries i0..i0+cnt (cnt <= waves): kl, and H as vectors
   int run(const scs_graph* g, u32 cnt, const u32* l, const u32* h, const u32* q, u32* kl,
@@ -357,7 +361,7 @@ struct HeuGpu {
     HIPCHK(hipMemcpyAsync(d_in, in.data(), sizeof(u32) * 3 * cnt, hipMemcpyHostToDevice, s));
     HIPCHK(scs_launch_heu(g->d_off32, (const u32*)g->d_adj, g->hg.n, g->hg.dmax, cnt, d_in,
                           d_in + cnt, d_in + 2 * cnt, ws, lay, SCS_MAX_H, d_out, d_out + cnt, d_H,
-                          s));
+                          d_out3, d_out3 + 3 * cnt, d_H3, s));
     HIPCHK(hipMemcpyAsync(h_out.data(), d_out, sizeof(u32) * 2 * cnt, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(h_H.data(), d_H, sizeof(u32) * (u64)SCS_MAX_H * cnt,
                           hipMemcpyDeviceToHost, s));
@@ -701,7 +705,7 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   if (p.gpu_heuristic && g->d_off32) {
     // as many waves as an admission step takes queries, within 4 GiB of
     // workspace (a launch lasts as long as its slowest query)
-    const u64 stride = scs_heu_layout(g->hg.n, SCS_MAX_H).stride;
+    const u64 stride = 3 * scs_heu_layout(g->hg.n, SCS_MAX_H).stride;
     const u32 w = (u32)std::max<u64>(16, std::min<u64>(std::max<u32>(p.admit_per_step, 64),
                                                        (4ull << 30) / std::max<u64>(stride, 1)));
     if (E->heu_gpu.init(g->hg.n, w) != SCS_OK) return fail(SCS_ENOMEM);

@@ -1,6 +1,7 @@
 // Greedy lower bounds on the GPU: the reference's CSSC_heu
 // (inc/heuristic.h:321-348), the best of its three greedy constructions
-// heu2 (:8-120), heu3 (:125-239) and heu4 (:242-317), one wave per query.
+// heu2 (:8-120), heu3 (:125-239) and heu4 (:242-317), one wave per
+// construction of each query.
 //
 // The reference drives each construction with a std::priority_queue that
 // holds stale duplicates.  Its pop order reduces to a deterministic choice
@@ -41,6 +42,9 @@ struct HeuArgs {
   u32* kl;     // nq
   u32* hsize;  // nq (sizes above hcap are reported, H is cut at hcap)
   u32* H;      // nq * hcap
+  u32* kl3;    // 3 * nq: heu2, heu3, heu4 of each query
+  u32* hs3;    // 3 * nq
+  u32* H3;     // 3 * nq * hcap
 };
 
 #define LANE (threadIdx.x & 63u)
@@ -541,12 +545,14 @@ __device__ __forceinline__ void heu4(HW& w, u32 N1, u32 N2, u32 q, u32* outH, u3
   cleanup(w);
 }
 
-// No LDS: the wave's two counters live in its workspace, so the kernel also
-// finds room on CUs whose LDS the search kernel holds.
+// Three waves per query, one per construction (block 3i + c runs heu2,
+// heu3 or heu4 of query i in its own workspace).  No LDS: the wave's
+// counter lives in its workspace, so the kernel also finds room on CUs
+// whose LDS the search kernel holds.
 __global__ void __launch_bounds__(64) __attribute__((flatten)) scs_heu_kernel(HeuArgs A) {
-  const u32 qi = blockIdx.x;
+  const u32 bi = blockIdx.x, qi = bi / 3, which = bi % 3;
   if (qi >= A.nq) return;
-  u8* base = A.ws + (u64)blockIdx.x * A.lay.stride;
+  u8* base = A.ws + (u64)bi * A.lay.stride;
   HW w;
   w.off = A.off;
   w.adj = A.adj;
@@ -568,35 +574,33 @@ __global__ void __launch_bounds__(64) __attribute__((flatten)) scs_heu_kernel(He
   __threadfence();
   hsync();
   const u32 l = A.ql[qi], h = A.qh[qi], q = A.qq[qi];
-  u32* out = A.H + (u64)qi * A.hcap;
-  // each construction writes its H to its own third of th's tail, then the
-  // best is copied out (CSSC_heu: heu4 on ties with both, heu3 over heu2)
-  u32* h2 = w.th + A.hcap;
-  u32* h3 = h2 + A.hcap;
-  u32* h4 = h3 + A.hcap;
-  u32 s2, k2, s3, k3, s4, k4;
-  heu2(w, l, h, q, h2, A.hcap, s2, k2);
-  heu3(w, l, h, q, h3, A.hcap, s3, k3);
-  heu4(w, l, h, q, h4, A.hcap, s4, k4);
-  hsync();
-  const u32* src;
-  u32 sz, kk;
-  if (k4 >= k3 && k4 >= k2) {
-    src = h4;
-    sz = s4;
-    kk = k4;
-  } else if (k3 >= k2) {
-    src = h3;
-    sz = s3;
-    kk = k3;
-  } else {
-    src = h2;
-    sz = s2;
-    kk = k2;
+  u32* out = A.H3 + (u64)bi * A.hcap;
+  u32 sz = 0, kk = 0;
+  if (which == 0)
+    heu2(w, l, h, q, out, A.hcap, sz, kk);
+  else if (which == 1)
+    heu3(w, l, h, q, out, A.hcap, sz, kk);
+  else
+    heu4(w, l, h, q, out, A.hcap, sz, kk);
+  if (LANE == 0) {
+    A.kl3[bi] = kk;
+    A.hs3[bi] = sz;
   }
+}
+
+// CSSC_heu's pick (inc/heuristic.h:321-348): heu4 when it ties with both,
+// else heu3 when it ties with heu2, else heu2.
+__global__ void __launch_bounds__(64) scs_heu_pick(HeuArgs A) {
+  const u32 qi = blockIdx.x;
+  if (qi >= A.nq) return;
+  const u32 k2 = A.kl3[3 * qi], k3 = A.kl3[3 * qi + 1], k4 = A.kl3[3 * qi + 2];
+  const u32 c = (k4 >= k3 && k4 >= k2) ? 2u : (k3 >= k2 ? 1u : 0u);
+  const u32 sz = A.hs3[3 * qi + c];
+  const u32* src = A.H3 + (u64)(3 * qi + c) * A.hcap;
+  u32* out = A.H + (u64)qi * A.hcap;
   for (u32 i = LANE; i < sz && i < A.hcap; i += 64) out[i] = src[i];
   if (LANE == 0) {
-    A.kl[qi] = kk;
+    A.kl[qi] = c == 2 ? k4 : (c == 1 ? k3 : k2);
     A.hsize[qi] = sz;
   }
 }
@@ -623,10 +627,12 @@ extern "C" HeuLayout scs_heu_layout(u32 n, u32 hcap) {
   return L;
 }
 
+// ws holds 3 * nq workspaces; kl3/hs3/H3 hold the three constructions'
+// results of each query, kl/hsize/H the picked one.
 extern "C" hipError_t scs_launch_heu(const u32* off, const u32* adj, u32 n, u32 dmax, u32 nq,
                                      const u32* ql, const u32* qh, const u32* qq, u8* ws,
                                      HeuLayout lay, u32 hcap, u32* kl, u32* hsize, u32* H,
-                                     hipStream_t s) {
+                                     u32* kl3, u32* hs3, u32* H3, hipStream_t s) {
   HeuArgs A;
   A.off = off;
   A.adj = adj;
@@ -642,6 +648,10 @@ extern "C" hipError_t scs_launch_heu(const u32* off, const u32* adj, u32 n, u32 
   A.kl = kl;
   A.hsize = hsize;
   A.H = H;
-  scs_heu_kernel<<<nq, 64, 0, s>>>(A);
+  A.kl3 = kl3;
+  A.hs3 = hs3;
+  A.H3 = H3;
+  scs_heu_kernel<<<3 * nq, 64, 0, s>>>(A);
+  scs_heu_pick<<<nq, 64, 0, s>>>(A);
   return hipGetLastError();
 }
