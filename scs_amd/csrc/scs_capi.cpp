@@ -703,11 +703,11 @@ extern "C" int scs_engine_create(scs_graph* g, const scs_params* pin, scs_engine
   E->h_slots.resize(nq);
   E->heu_work.resize(p.heuristic_threads);
   if (p.gpu_heuristic && g->d_off32) {
-    // as many waves as an admission step takes queries, within 4 GiB of
-    // workspace (a launch lasts as long as its slowest query)
+    // as many queries per launch as an admission step takes, within 16 GiB
+    // of workspace (a launch lasts as long as its slowest query)
     const u64 stride = 3 * scs_heu_layout(g->hg.n, SCS_MAX_H).stride;
     const u32 w = (u32)std::max<u64>(16, std::min<u64>(std::max<u32>(p.admit_per_step, 64),
-                                                       (4ull << 30) / std::max<u64>(stride, 1)));
+                                                       (16ull << 30) / std::max<u64>(stride, 1)));
     if (E->heu_gpu.init(g->hg.n, w) != SCS_OK) return fail(SCS_ENOMEM);
   }
   E->st.waves = E->nwaves;
