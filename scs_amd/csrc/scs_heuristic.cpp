@@ -9,7 +9,9 @@
 //   heu3 (inc/heuristic.h:125-239): the connection score sums 1/deg over the
 //     vertex's neighbours in H; each vertex keeps its H-neighbours in
 //     ascending order, so the sum has the same terms in the same order as
-//     the reference's adjacency scan (bit-identical doubles);
+//     the reference's adjacency scan (bit-identical doubles); a push no
+//     larger than the vertex's largest earlier key is dropped, since it
+//     could never be popped before that one;
 //   heu4 (inc/heuristic.h:242-317): unchanged except that q's +1
 //     (inc/heuristic.h:259) is not counted in the min-degree test when q is
 //     popped alone (l <= 1); the reference then reports 1 for an isolated q.
@@ -32,6 +34,7 @@ void HeuWork::reset(uint32_t n) {
     deg.assign(n, 0);
     cnt.assign(n, 0);
     thn.assign(n, {});
+    key3.assign(n, 0.0);
     inS.assign(n, 0);
   }
 }
@@ -42,6 +45,7 @@ void HeuWork::clear_touched() {
     deg[v] = 0;
     cnt[v] = 0;
     thn[v].clear();
+    key3[v] = 0.0;
     inS[v] = 0;
   }
   touched.clear();
@@ -149,7 +153,12 @@ static void heu3(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
       for (uint32_t w : W.thn[nb])
         if (W.deg[w] != 0) score += (double)1 / W.deg[w];
       score += (double)(g.off[nb + 1] - g.off[nb]) / dMAX;
-      Q.push({score, nb});
+      // a push no larger than nb's largest earlier key is never the first
+      // of nb's entries popped, so it cannot change the pop sequence
+      if (S(nb) == 0 || score > W.key3[nb]) {
+        W.key3[nb] = score;
+        Q.push({score, nb});
+      }
       if (S(nb) == 0) setS(nb, 1);
     }
     if (tH.size() >= N1) {

@@ -19,6 +19,7 @@ struct HeuWork {
   std::vector<uint8_t> sta;  // bit 7 = touched, low bits = reference status
   std::vector<uint32_t> deg, cnt;
   std::vector<std::vector<uint32_t>> thn;
+  std::vector<double> key3;  // heu3: largest score pushed for each vertex
   std::vector<uint8_t> inS;
   std::vector<uint32_t> touched;
   void reset(uint32_t n);
