@@ -87,15 +87,17 @@ def _c1_retry(i):
     return r
 
 
-def retry_capped(jobs, cap):
-    """Second pass: the C1 entries still CAPPED in golden.json are solved again
-    with node cap `cap`; the proven ones replace their entries (each entry
-    records the cap it was solved with)."""
+def retry_capped(jobs, cap, only=None):
+    """Second pass: the C1 entries still CAPPED in golden.json (or those of
+    them listed in `only`) are solved again with node cap `cap`; the proven
+    ones replace their entries (each entry records the cap it was solved
+    with)."""
     import time
     from multiprocessing import Pool
     path = os.path.join(HERE, "golden.json")
     out = json.load(open(path))
-    todo = [r["index"] for r in out["C1"] if r["status"] == "CAPPED"]
+    todo = [r["index"] for r in out["C1"] if r["status"] == "CAPPED"
+            and (only is None or r["index"] in only)]
     _RETRY_CAP[0] = cap
     t0 = time.time()
     done = proven = 0
@@ -155,8 +157,11 @@ if __name__ == "__main__":
     ap.add_argument("--jobs", type=int, default=8)
     ap.add_argument("--retry-cap", type=int, default=0,
                     help="only re-solve the C1 entries still CAPPED, with this node cap")
+    ap.add_argument("--only", default="",
+                    help="with --retry-cap: comma-separated C1 indices to retry")
     a = ap.parse_args()
     if a.retry_cap:
-        retry_capped(a.jobs, a.retry_cap)
+        only = {int(x) for x in a.only.split(",") if x} or None
+        retry_capped(a.jobs, a.retry_cap, only)
     else:
         main(a.jobs)
