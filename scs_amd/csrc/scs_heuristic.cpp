@@ -8,10 +8,11 @@
 //     running count instead of rescanning its adjacency;
 //   heu3 (inc/heuristic.h:125-239): the connection score sums 1/deg over the
 //     vertex's neighbours in H; each vertex keeps its H-neighbours in
-//     ascending order, so the sum has the same terms in the same order as
-//     the reference's adjacency scan (bit-identical doubles); a push no
-//     larger than the vertex's largest earlier key is dropped, since it
-//     could never be popped before that one;
+//     ascending order (in a 32-byte record with its largest pushed key), so
+//     the sum has the same terms in the same order as the reference's
+//     adjacency scan (bit-identical doubles); a push no larger than the
+//     vertex's largest earlier key is dropped, since it could never be
+//     popped before that one;
 //   heu4 (inc/heuristic.h:242-317): unchanged except that q's +1
 //     (inc/heuristic.h:259) is not counted in the min-degree test when q is
 //     popped alone (l <= 1); the reference then reports 1 for an isolated q.
@@ -33,8 +34,7 @@ void HeuWork::reset(uint32_t n) {
     sta.assign(n, 0);
     deg.assign(n, 0);
     cnt.assign(n, 0);
-    thn.assign(n, {});
-    key3.assign(n, 0.0);
+    rec3.assign(n, Heu3Rec{});
     inS.assign(n, 0);
   }
 }
@@ -44,8 +44,8 @@ void HeuWork::clear_touched() {
     sta[v] = 0;
     deg[v] = 0;
     cnt[v] = 0;
-    thn[v].clear();
-    key3[v] = 0.0;
+    rec3[v].key = 0.0;
+    rec3[v].len = 0;
     inS[v] = 0;
   }
   touched.clear();
@@ -82,13 +82,11 @@ static void heu2(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
     tH.push_back(v);
     setS(v, 2);
     const int64_t b = g.off[v], e = g.off[v + 1];
+    // neighbours are distinct, so each count is final once incremented
     for (int64_t j = b; j < e; j++) {
       uint32_t nb = (uint32_t)g.adj[j];
       touch(W, nb);
       W.cnt[nb]++;
-    }
-    for (int64_t j = b; j < e; j++) {
-      uint32_t nb = (uint32_t)g.adj[j];
       if (S(nb) != 2) {
         Q.push({W.cnt[nb], nb});
         setS(nb, 1);
@@ -112,10 +110,39 @@ static void heu2(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
   W.clear_touched();
 }
 
+// Inserts v into r's ascending list and returns the list.  A list longer
+// than three lives in W.pool3 and moves to twice the room when its length
+// reaches a power of two >= 8; the pool is emptied at the start of each heu3
+// run.
+static inline uint32_t* insert3(HeuWork& W, Heu3Rec& r, uint32_t v) {
+  uint32_t* L = r.len <= 3 ? r.inl : W.pool3.data() + r.ext;
+  if (r.len == 3 || (r.len >= 8 && (r.len & (r.len - 1)) == 0)) {
+    const size_t cap = r.len == 3 ? 8 : 2 * (size_t)r.len;
+    if (W.top3 + cap > W.pool3.size()) {
+      W.pool3.resize(std::max(2 * W.pool3.size(), W.top3 + cap));
+      if (r.len > 3) L = W.pool3.data() + r.ext;
+    }
+    uint32_t* M = W.pool3.data() + W.top3;
+    std::memcpy(M, L, r.len * sizeof(uint32_t));
+    r.ext = W.top3;
+    W.top3 += cap;
+    L = M;
+  }
+  uint32_t i = r.len;
+  while (i > 0 && L[i - 1] > v) {
+    L[i] = L[i - 1];
+    i--;
+  }
+  L[i] = v;
+  r.len++;
+  return L;
+}
+
 static void heu3(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWork& W,
                  std::vector<uint32_t>& H, uint32_t& kl) {
   H.clear();
   kl = 0;
+  W.top3 = 0;
   std::vector<uint32_t> tH;
   uint32_t tkl = 0, tsize = 0;
   std::priority_queue<std::pair<double, uint32_t>> Q;
@@ -139,24 +166,23 @@ static void heu3(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
         ++W.deg[v];
       }
     }
-    // v joins the ascending H-neighbour list of each neighbour
+    // v joins the ascending H-neighbour list of each neighbour outside H
+    // (lists of H members are never read again); all degree updates are
+    // done above, so each neighbour's score is final once v is in its list
     for (int64_t j = b; j < e; j++) {
       uint32_t nb = (uint32_t)g.adj[j];
       touch(W, nb);
-      auto& L = W.thn[nb];
-      L.insert(std::upper_bound(L.begin(), L.end(), v), v);
-    }
-    for (int64_t j = b; j < e; j++) {
-      uint32_t nb = (uint32_t)g.adj[j];
       if (S(nb) == 2) continue;
+      Heu3Rec& r = W.rec3[nb];
+      uint32_t* L = insert3(W, r, v);
       double score = 0;
-      for (uint32_t w : W.thn[nb])
-        if (W.deg[w] != 0) score += (double)1 / W.deg[w];
+      for (uint32_t i = 0; i < r.len; i++)
+        if (W.deg[L[i]] != 0) score += (double)1 / W.deg[L[i]];
       score += (double)(g.off[nb + 1] - g.off[nb]) / dMAX;
       // a push no larger than nb's largest earlier key is never the first
       // of nb's entries popped, so it cannot change the pop sequence
-      if (S(nb) == 0 || score > W.key3[nb]) {
-        W.key3[nb] = score;
+      if (S(nb) == 0 || score > r.key) {
+        r.key = score;
         Q.push({score, nb});
       }
       if (S(nb) == 0) setS(nb, 1);

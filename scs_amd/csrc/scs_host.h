@@ -13,13 +13,26 @@ struct HostGraph {
   std::vector<uint32_t> core;
 };
 
+// heu3's state of one vertex, in one 32-byte record so that a visit of a
+// neighbour touches one cache line: the largest score pushed for it, and its
+// H-neighbours in ascending order (the first three in place; a longer list
+// at offset ext of the workspace's pool, in room for max(8, the next power of
+// two) entries).
+struct Heu3Rec {
+  double key;
+  uint64_t ext;
+  uint32_t len;
+  uint32_t inl[3];
+};
+
 // Per-thread scratch of the heuristic; n-sized arrays are reused across
 // queries and only touched entries are cleared.
 struct HeuWork {
   std::vector<uint8_t> sta;  // bit 7 = touched, low bits = reference status
   std::vector<uint32_t> deg, cnt;
-  std::vector<std::vector<uint32_t>> thn;
-  std::vector<double> key3;  // heu3: largest score pushed for each vertex
+  std::vector<Heu3Rec> rec3;
+  std::vector<uint32_t> pool3;  // heu3: the longer H-neighbour lists of one run
+  size_t top3 = 0;              // first free entry of pool3
   std::vector<uint8_t> inS;
   std::vector<uint32_t> touched;
   void reset(uint32_t n);

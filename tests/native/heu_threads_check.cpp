@@ -3,7 +3,8 @@
 // read-only graph, each with its own scratch, writing their answers into
 // per-query slots, as scs_step does.  Built with -fsanitize=thread (or
 // address,undefined) by tests/test_host_sanitizers_cpu.py; the answers must
-// equal a single-threaded pass.
+// equal a single-threaded pass, which in turn must equal runs with a fresh
+// scratch per query.
 //
 // Input (argv[1]): a binary CSR, n (u32), nnz (u64), off[n+1] (i64),
 // adj[nnz] (i32).  argv[2]: number of queries, argv[3]: threads.
@@ -41,12 +42,22 @@ int main(int argc, char** argv) {
     x.h = x.l + 2 + rng() % 20;
     x.q = rng() % g.n;
   }
-  // single-threaded answers
+  // single-threaded answers, one scratch reused across the queries
   std::vector<uint32_t> kl1(nq);
   std::vector<std::vector<uint32_t>> H1(nq);
   {
     scs::HeuWork W;
     for (uint32_t i = 0; i < nq; i++) kl1[i] = scs::cssc_heu(g, qs[i].l, qs[i].h, qs[i].q, W, H1[i]);
+  }
+  // a fresh scratch per query gives the same answers: clearing the touched
+  // entries leaves nothing of the previous query behind
+  for (uint32_t i = 0; i < nq; i += 7) {
+    scs::HeuWork W;
+    std::vector<uint32_t> H0;
+    if (scs::cssc_heu(g, qs[i].l, qs[i].h, qs[i].q, W, H0) != kl1[i] || H0 != H1[i]) {
+      fprintf(stderr, "query %u differs with a fresh scratch\n", i);
+      return 1;
+    }
   }
   // threads claiming queries from one counter, as the admission step does
   std::vector<uint32_t> kl(nq);

@@ -51,13 +51,32 @@ def host_heuristic(lib, off, adj, l, h, q):
     return kl.value, H[:hs.value]
 
 
-@pytest.mark.parametrize("gname", ["100.txt", "hepPH_SCS", "C1"])
+def dense_graph(n, p, seed):
+    """G(n, p) as a CSR (sorted rows, symmetric, no self-loops)."""
+    rng = np.random.default_rng(seed)
+    a = np.triu(rng.random((n, n)) < p, 1)
+    a = a | a.T
+    off = np.concatenate([[0], np.cumsum(a.sum(1))]).astype(np.int64)
+    adj = np.nonzero(a)[1].astype(np.int32)
+    return OracleGraph.from_csr(n, off, adj)
+
+
+@pytest.mark.parametrize("gname", ["100.txt", "hepPH_SCS", "C1", "dense"])
 def test_host_heuristic_equals_oracle(lib, gname):
     if gname == "C1":
         g = configs.graph("C1")
         og = OracleGraph.from_csr(g.n, g.off, g.adj)
         core = og.core()
         qs = [(x.l, x.h, x.q) for x in configs.queries("C1", core.astype(np.int64))[:40]]
+    elif gname == "dense":
+        # vertices with up to h neighbours in H: heu3's neighbour lists grow
+        # past their in-place room and through several doublings
+        og = dense_graph(300, 0.4, 5)
+        rng = np.random.default_rng(4)
+        qs = []
+        for _ in range(20):
+            l = int(rng.integers(2, 60))
+            qs.append((l, l + int(rng.integers(0, 80)), int(rng.integers(0, og.n))))
     else:
         og = OracleGraph.load(os.path.join(GOLD, gname))
         rng = np.random.default_rng(3)
