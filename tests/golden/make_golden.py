@@ -91,7 +91,7 @@ def retry_capped(jobs, cap, only=None):
     """Second pass: the C1 entries still CAPPED in golden.json (or those of
     them listed in `only`) are solved again with node cap `cap`; the proven
     ones replace their entries (each entry records the cap it was solved
-    with)."""
+    with), and the pass appends its record to C1_search.retries."""
     import time
     from multiprocessing import Pool
     path = os.path.join(HERE, "golden.json")
@@ -101,6 +101,11 @@ def retry_capped(jobs, cap, only=None):
     _RETRY_CAP[0] = cap
     t0 = time.time()
     done = proven = 0
+    rec = {"node_cap": cap, "queries": len(todo), "answered": 0, "jobs": jobs,
+           "wall_seconds": 0.0, "proven": 0}
+    if only is not None:
+        rec["only"] = sorted(only)
+    out["C1_search"].setdefault("retries", []).append(rec)
     with Pool(jobs, initializer=_c1_init) as p:
         # the file is rewritten after every answer, so a pass cut short keeps
         # what it proved
@@ -110,9 +115,7 @@ def retry_capped(jobs, cap, only=None):
             if r["status"] != "CAPPED":
                 out["C1"][r["index"]] = r
                 proven += 1
-            out["C1_search"]["retry"] = {"node_cap": cap, "queries": len(todo), "answered": done,
-                                         "jobs": jobs, "wall_seconds": round(time.time() - t0, 1),
-                                         "proven": proven}
+            rec.update(answered=done, wall_seconds=round(time.time() - t0, 1), proven=proven)
             out["C1_search"]["proven"] = sum(x["status"] != "CAPPED" for x in out["C1"])
             tmp = path + ".tmp"
             with open(tmp, "w") as f:
