@@ -28,6 +28,10 @@
 namespace scs {
 
 static const uint32_t INF_DEG = 1000000000u;
+// The neighbour loops of heu2 and heu3 are bound by cache misses on the
+// n-sized per-vertex arrays; they prefetch the entries of the neighbour this
+// many places ahead in the adjacency list.
+static const int64_t PF_AHEAD = 16;
 
 void HeuWork::reset(uint32_t n) {
   if (sta.size() != n) {
@@ -85,6 +89,7 @@ static void heu2(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
     // neighbours are distinct, so each count is final once incremented
     for (int64_t j = b; j < e; j++) {
       uint32_t nb = (uint32_t)g.adj[j];
+      if (j + PF_AHEAD < e) __builtin_prefetch(&W.cnt[(uint32_t)g.adj[j + PF_AHEAD]], 1);
       touch(W, nb);
       W.cnt[nb]++;
       if (S(nb) != 2) {
@@ -171,6 +176,11 @@ static void heu3(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
     // done above, so each neighbour's score is final once v is in its list
     for (int64_t j = b; j < e; j++) {
       uint32_t nb = (uint32_t)g.adj[j];
+      if (j + PF_AHEAD < e) {
+        const uint32_t f = (uint32_t)g.adj[j + PF_AHEAD];
+        __builtin_prefetch(&W.rec3[f], 1);
+        __builtin_prefetch(&g.off[f], 0);
+      }
       touch(W, nb);
       if (S(nb) == 2) continue;
       Heu3Rec& r = W.rec3[nb];
