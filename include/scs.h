@@ -214,13 +214,15 @@ typedef struct scs_stats {
 int scs_engine_stats(const scs_engine* e, scs_stats* s);
 
 /* Host-only: the greedy lower bound of the reference's CSSC_heu
- * (inc/heuristic.h:321-348) on a CSR; writes kl and H (up to cap ids). */
+ * (inc/heuristic.h:321-348) on a CSR; writes kl and H (up to cap ids).
+ * SCS_EINVAL for q >= n, h == 0, l > h, offsets that do not start at 0 or
+ * decrease, or adjacency missing or holding ids outside [0, n). */
 int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_t n, uint32_t l,
                       uint32_t h, uint32_t q, uint32_t* kl, uint32_t* H, uint32_t cap,
                       uint32_t* hsize);
 
 /* The same greedy lower bound (inc/heuristic.h:8-348) for `count` queries on
- * the GPU, one wave per query, on a resident graph: kl[i], hsize[i] (the size
+ * the GPU, one wave per construction of each query, on a resident graph: kl[i], hsize[i] (the size
  * of H, also when it exceeds cap) and H row i (up to cap ids).  Equal to
  * scs_heuristic_csr value for value.  SCS_EBIG when the graph has 2^32
  * adjacency entries or more. */

@@ -1426,7 +1426,12 @@ extern "C" int scs_engine_stats(const scs_engine* E, scs_stats* s) {
 extern "C" int scs_heuristic_csr(const int64_t* off, const int32_t* adj, uint32_t n, uint32_t l,
                                  uint32_t h, uint32_t q, uint32_t* kl, uint32_t* H, uint32_t cap,
                                  uint32_t* hsize) {
-  if (!off || !kl || q >= n || h == 0) return SCS_EINVAL;
+  if (!off || !kl || q >= n || h == 0 || l > h) return SCS_EINVAL;
+  if (off[0] != 0 || off[n] < 0 || (off[n] && !adj)) return SCS_EINVAL;
+  for (u32 v = 0; v < n; v++)
+    if (off[v + 1] < off[v]) return SCS_EINVAL;
+  for (int64_t j = 0; j < off[n]; j++)
+    if (adj[j] < 0 || (u32)adj[j] >= n) return SCS_EINVAL;
   scs::HostGraph g;
   g.n = n;
   g.off.assign(off, off + n + 1);

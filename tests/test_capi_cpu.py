@@ -92,6 +92,26 @@ def test_host_heuristic_equals_oracle(lib, gname):
         assert list(a[1]) == list(b[1]), (gname, l, h, q)
 
 
+def test_host_heuristic_rejects_bad_input(lib):
+    """scs_heuristic_csr returns SCS_EINVAL (-1) instead of reading outside
+    the caller's arrays."""
+    f = lib.scs_heuristic_csr
+    f.argtypes = [np.ctypeslib.ndpointer(np.int64, flags="C"), np.ctypeslib.ndpointer(np.int32, flags="C"),
+                  C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
+                  C.c_void_p, C.c_uint32, C.c_void_p]
+    kl = C.c_uint32()
+    off = np.array([0, 1, 2], np.int64)
+    adj = np.array([1, 0], np.int32)
+    assert f(off, adj, 2, 1, 2, 0, C.byref(kl), None, 0, None) == 0 and kl.value == 1
+    assert f(off, adj, 2, 1, 2, 2, C.byref(kl), None, 0, None) == -1      # q >= n
+    assert f(off, adj, 2, 3, 2, 0, C.byref(kl), None, 0, None) == -1      # l > h
+    assert f(off, adj, 2, 0, 0, 0, C.byref(kl), None, 0, None) == -1      # h == 0
+    assert f(off, np.array([1, 2], np.int32), 2, 1, 2, 0, C.byref(kl), None, 0, None) == -1
+    assert f(off, np.array([1, -1], np.int32), 2, 1, 2, 0, C.byref(kl), None, 0, None) == -1
+    assert f(np.array([0, 2, 1], np.int64), adj, 2, 1, 2, 0, C.byref(kl), None, 0, None) == -1
+    assert f(np.array([1, 1, 2], np.int64), adj, 2, 1, 2, 0, C.byref(kl), None, 0, None) == -1
+
+
 def test_ctypes_structs_match_the_header(tmp_path):
     """The Python binding's structures have the size and field offsets the C
     header gives (a program built with g++ against include/scs.h prints
