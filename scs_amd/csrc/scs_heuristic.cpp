@@ -43,14 +43,20 @@ void HeuWork::reset(uint32_t n) {
   }
 }
 
-void HeuWork::clear_touched() {
+void HeuWork::clear_touched(unsigned what) {
+  // every entry cleared here is a random access into an n-sized array, so
+  // only the arrays the finished construction wrote are visited
   for (uint32_t v : touched) {
     sta[v] = 0;
-    deg[v] = 0;
-    cnt[v] = 0;
-    rec3[v].key = 0.0;
-    rec3[v].len = 0;
-    inS[v] = 0;
+    if (what & CLEAR_CNT) cnt[v] = 0;
+    if (what & CLEAR_REC3) {
+      rec3[v].key = 0.0;
+      rec3[v].len = 0;
+    }
+    if (what & CLEAR_DEG_INS) {
+      deg[v] = 0;
+      inS[v] = 0;
+    }
   }
   touched.clear();
 }
@@ -112,7 +118,8 @@ static void heu2(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
   }
   H.assign(tH.begin(), tH.begin() + tsize);
   kl = tkl;
-  W.clear_touched();
+  for (uint32_t x : tH) W.deg[x] = 0;  // deg is written only at H members
+  W.clear_touched(HeuWork::CLEAR_CNT);
 }
 
 // Inserts v into r's ascending list and returns the list.  A list longer
@@ -209,7 +216,8 @@ static void heu3(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
   }
   H.assign(tH.begin(), tH.begin() + tsize);
   kl = tkl;
-  W.clear_touched();
+  for (uint32_t x : tH) W.deg[x] = 0;  // deg is written only at H members
+  W.clear_touched(HeuWork::CLEAR_REC3);
 }
 
 static void heu4(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWork& W,
@@ -269,7 +277,7 @@ static void heu4(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
   for (uint32_t x : S)
     if (W.inS[x]) H.push_back(x);
   kl = mindeg;
-  W.clear_touched();
+  W.clear_touched(HeuWork::CLEAR_DEG_INS);
 }
 
 uint32_t cssc_heu(const HostGraph& g, uint32_t l, uint32_t h, uint32_t q, HeuWork& W,

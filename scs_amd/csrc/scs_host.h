@@ -36,7 +36,11 @@ struct HeuWork {
   std::vector<uint8_t> inS;
   std::vector<uint32_t> touched;
   void reset(uint32_t n);
-  void clear_touched();
+  // Zeroes sta and the arrays named in `what` at every touched vertex; each
+  // construction names only the arrays it wrote at vertices outside its H
+  // list (heu2 and heu3 zero deg over their H list themselves).
+  enum : unsigned { CLEAR_CNT = 1, CLEAR_REC3 = 2, CLEAR_DEG_INS = 4 };
+  void clear_touched(unsigned what);
   bool mark_touched(uint32_t v);
 };
 
