@@ -171,12 +171,14 @@ static void heu3(const HostGraph& g, uint32_t N1, uint32_t N2, uint32_t q, HeuWo
     tH.push_back(v);
     setS(v, 2);
     const int64_t b = g.off[v], e = g.off[v + 1];
-    for (int64_t j = b; j < e; j++) {
-      uint32_t nb = (uint32_t)g.adj[j];
-      if (S(nb) == 2) {
-        ++W.deg[nb];
-        ++W.deg[v];
-      }
+    {
+      // v's list holds exactly its neighbours already in H (each joined H
+      // while v was outside it), so the degree updates need no scan of v's
+      // adjacency
+      const Heu3Rec& rv = W.rec3[v];
+      const uint32_t* Lv = rv.len <= 3 ? rv.inl : W.pool3.data() + rv.ext;
+      W.deg[v] += rv.len;
+      for (uint32_t i = 0; i < rv.len; i++) ++W.deg[Lv[i]];
     }
     // v joins the ascending H-neighbour list of each neighbour outside H
     // (lists of H members are never read again); all degree updates are
