@@ -10,9 +10,10 @@
 //     vertex's neighbours in H; each vertex keeps its H-neighbours in
 //     ascending order (in a 32-byte record with its largest pushed key), so
 //     the sum has the same terms in the same order as the reference's
-//     adjacency scan (bit-identical doubles); a push no larger than the
-//     vertex's largest earlier key is dropped, since it could never be
-//     popped before that one;
+//     adjacency scan (bit-identical doubles), and a joining vertex's degree
+//     into H is its list's length; a push no larger than the vertex's
+//     largest earlier key is dropped, since it could never be popped before
+//     that one;
 //   heu4 (inc/heuristic.h:242-317): unchanged except that q's +1
 //     (inc/heuristic.h:259) is not counted in the min-degree test when q is
 //     popped alone (l <= 1); the reference then reports 1 for an isolated q.
